@@ -1,0 +1,208 @@
+/*
+ * gofish_capi.h -- C ABI of the MI355X (gfx950) GO_Fish / Spectrum library.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, int status codes.
+ * No C++ or GPU-runtime types appear in any signature, so the same header is
+ * consumed by the C++ API layer (go_fish.cuh, spectrum.h), by ctypes, and by
+ * any other FFI.  Every entry point returns GF_OK (0) or a negative GF_ERR_*
+ * code; gf_last_error() returns the message of the last failure on the
+ * calling thread.
+ *
+ * The reference (ParallelPopGen 0.3.2) has no C ABI: its surface is the C++
+ * template API that go_fish.cuh/spectrum.h re-declare.  Each entry point
+ * below names the reference function(s) whose work it takes over.
+ */
+#ifndef GOFISH_CAPI_H_
+#define GOFISH_CAPI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----- status codes ----- */
+#define GF_OK              0
+#define GF_ERR_ARG        -1   /* bad argument (null pointer, size out of range) */
+#define GF_ERR_PARAM      -2   /* simulation parameter check failed (check_sim_parameters) */
+#define GF_ERR_DEVICE     -3   /* HIP runtime error */
+#define GF_ERR_ALLOC      -4   /* device or host allocation failed */
+#define GF_ERR_UNSUPPORTED -5  /* valid request this build does not implement */
+
+/* Maximum number of populations one simulation may hold (register-resident
+ * per-mutation state in the drift kernel). */
+#define GF_MAX_POPULATIONS 8
+
+/* Mutation identifier; same layout and meaning as GO_Fish::mutID
+ * (reference 3P/go_fish_data_struct.h:35-51).  A negative origin_threadID
+ * marks the mutation as preserved. */
+typedef struct gf_mutid {
+    int32_t origin_generation;
+    int32_t origin_population;
+    int32_t origin_threadID;
+    int32_t reserved;
+} gf_mutid;
+
+/* Value of every host-side model function for one population at one
+ * generation.  The selection model is carried in its linear form
+ * s(x) = sel_slope * x + sel_intercept, which every Sim_Model selection
+ * functor reduces to for a fixed (population, generation)
+ * (reference 3P/_internal/template_inline_simulation_functors.cuh:46-158). */
+typedef struct gf_pop_gen {
+    int32_t N;              /* demography(pop, gen): number of individuals   */
+    float   F;              /* inbreeding coefficient FI(pop, gen)           */
+    float   h;              /* dominance(pop, gen)                           */
+    float   mu;             /* per-site mutation rate mu_rate(pop, gen)      */
+    float   sel_slope;      /* selection: s(pop, gen, x) = slope*x + intercept */
+    float   sel_intercept;
+} gf_pop_gen;
+
+/* Initialisation modes (reference run_sim, go_fish_impl.cuh:593-623). */
+#define GF_INIT_MSE       0   /* mutation-selection equilibrium            */
+#define GF_INIT_BLANK     1   /* empty mutation array                      */
+#define GF_INIT_PREV      2   /* arrays of a previous run's time sample    */
+
+/* Full description of one run_sim call.  Tables are indexed by
+ * t = generation - start_generation for t in [0, num_generations]. */
+typedef struct gf_sim_desc {
+    int32_t num_populations;      /* P, 1..GF_MAX_POPULATIONS */
+    float   num_sites;            /* L */
+    int32_t seed1, seed2;         /* Philox key */
+    int32_t num_generations;      /* generations to simulate after the start */
+    int32_t compact_interval;     /* 0 disables compaction */
+    int32_t init_mode;            /* GF_INIT_* */
+    int32_t start_generation;     /* 0 unless GF_INIT_PREV (sampled generation of prev run) */
+    /* GF_INIT_PREV only: */
+    int32_t        init_num_mutations;
+    const float   *init_freq;     /* [P][init_num_mutations] row-major */
+    const gf_mutid*init_ids;      /* [init_num_mutations] */
+    const uint8_t *init_extinct;  /* [P] */
+    /* model tables */
+    const gf_pop_gen *params;     /* [(num_generations+1)][P] */
+    const float      *migration;  /* [(num_generations+1)][P from][P to] */
+    const uint8_t    *preserve;   /* [(num_generations+1)] preserve_mutations(g) */
+    const uint8_t    *sample;     /* [(num_generations+1)] take_sample(g) */
+    int32_t device;               /* HIP device ordinal, -1 = current */
+} gf_sim_desc;
+
+typedef struct gf_result gf_result;
+
+/* Kernel timing collected while gf_run executes (only when enabled with
+ * gf_set_kernel_timing(1)); durations in milliseconds from HIP events
+ * recorded on the library's stream around each launch. */
+typedef struct gf_timing {
+    double  advance_ms;        /* migration_selection_drift + add_new_mutations */
+    int64_t advance_launches;
+    double  compact_ms;        /* flag + scan + scatter */
+    int64_t compact_launches;
+    double  init_ms;           /* mutation-selection-equilibrium initialisation */
+    double  total_ms;          /* whole gf_run, host clock */
+    int64_t advance_slot_generations;  /* slot x generation updates issued by the advance kernel */
+} gf_timing;
+
+/* Runs one simulation (replaces GO_Fish::run_sim, go_fish_impl.cuh:553-730,
+ * and its helpers initialize_mse :242-332, init_blank_prev_run :334-368,
+ * compact :370-429, calc_new_mutations_Index :454-467,
+ * store_time_sample :477-501; kernels migration_selection_drift :93-124,
+ * add_new_mutations go_fish_impl.cu:138-145, flag_segregating_mutations
+ * :137-176, scatter_arrays go_fish_impl.cu:147-189). */
+int32_t gf_run(const gf_sim_desc *desc, gf_result **out);
+
+/* Result accessors (reference allele_trajectories accessors,
+ * inline_go_fish_data_struct.hpp:263-323). */
+int32_t gf_result_num_samples(const gf_result *r);
+int32_t gf_result_sample_info(const gf_result *r, int32_t sample,
+                              int32_t *num_mutations, int32_t *generation,
+                              int32_t *Nchrom_e /*[P]*/, uint8_t *extinct /*[P]*/);
+int32_t gf_result_sample_freq(const gf_result *r, int32_t sample, float *out /*[P][M]*/);
+int32_t gf_result_num_ids(const gf_result *r);
+int32_t gf_result_ids(const gf_result *r, gf_mutid *out);
+/* live (segregating or preserved) mutation count after each generation's
+ * update, t = 0..num_generations (t = 0: the initial state) */
+int32_t gf_result_live_counts(const gf_result *r, int64_t *out /*[num_generations+1]*/);
+int32_t gf_result_timing(const gf_result *r, gf_timing *out);
+void    gf_result_free(gf_result *r);
+
+/* Population frequency histogram of one population row
+ * (replaces Spectrum::population_frequency_histogram, spectrum.cu:153-210,
+ * kernels population_hist :74-83 and uint_to_float :85-88).
+ * freq: host pointer, M values.  out: host pointer, Nchrom_e floats. */
+int32_t gf_population_histogram(const float *freq, int32_t num_mutations,
+                                int32_t Nchrom_e, float num_sites,
+                                float *out, int32_t device);
+
+/* Expected site frequency spectrum of a sample of n chromosomes
+ * (replaces Spectrum::site_frequency_spectrum, spectrum.cu:213-294,
+ * kernels binom_coeff :90-95 and binom :111-139).  Accumulation is in
+ * 2^-32 fixed point, so the result does not depend on thread order.
+ * freq: host pointer, M values.  out: host pointer, n floats. */
+int32_t gf_site_frequency_spectrum(const float *freq, int32_t num_mutations,
+                                   int32_t sample_size, float num_sites,
+                                   float *out, int32_t device);
+
+/* ----- batched simulation (no reference counterpart: replaces the
+ * back-to-back run_sim + site_frequency_spectrum loop of
+ * examples/example_dadi/run.cu:139-150) ----- */
+typedef struct gf_batch_sim {
+    int32_t N;                 /* individuals, constant in time */
+    float   F, h, mu;
+    float   s;                 /* constant selection coefficient */
+    float   num_sites;
+    int32_t seed1, seed2;
+    int32_t num_generations;
+} gf_batch_sim;
+
+typedef struct gf_batch_result gf_batch_result;
+
+/* Runs every simulation from mutation-selection equilibrium to its own final
+ * generation, one population each, and reports the final SFS of a sample of
+ * sfs_n chromosomes per simulation.  Per-simulation results are identical
+ * to gf_run on the equivalent gf_sim_desc. */
+int32_t gf_batch_run(const gf_batch_sim *sims, int32_t num_sims,
+                     int32_t compact_interval, int32_t sfs_n,
+                     int32_t device, gf_batch_result **out);
+int32_t gf_batch_sfs(const gf_batch_result *r, float *out /*[num_sims][sfs_n]*/);
+int32_t gf_batch_num_mutations(const gf_batch_result *r, int32_t *out /*[num_sims]*/);
+/* final frequencies and IDs of one simulation, compacted; out arrays sized
+ * by gf_batch_num_mutations */
+int32_t gf_batch_sim_state(const gf_batch_result *r, int32_t sim, float *freq, gf_mutid *ids);
+int32_t gf_batch_live_mutation_generations(const gf_batch_result *r, int64_t *out);
+int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out);
+void    gf_batch_free(gf_batch_result *r);
+
+/* ----- diagnostics (no reference counterpart) -----
+ * Evaluates the device implementation of one random-number / elementary
+ * function on n inputs (host arrays in, host array out), for checking it
+ * bit for bit against the host restatement.
+ *   PHILOX     in0: uint32 [n][6] = key0 key1 c0 c1 c2 c3   out: uint32 [n][4]
+ *   U01        in0: uint32                                  out: float
+ *   EXP, LOG, NORMCDFINV, SQRT, ROUND_AWAY: in0 float       out: float
+ *   DIV        in0 / in1 (float)                             out: float
+ *   BINOM_DRAW in0 uint32 bits, in1 mean, in2 var, in3 N     out: int32
+ *   POIS_DRAW  in0 uint32 bits, in1 mean, in2 N              out: int32 */
+#define GF_MATH_PHILOX      0
+#define GF_MATH_U01         1
+#define GF_MATH_EXP         2
+#define GF_MATH_LOG         3
+#define GF_MATH_NORMCDFINV  4
+#define GF_MATH_BINOM_DRAW  5
+#define GF_MATH_POIS_DRAW   6
+#define GF_MATH_SQRT        7
+#define GF_MATH_DIV         8
+#define GF_MATH_ROUND_AWAY  9
+int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void *in2, const void *in3,
+                       void *out, int64_t n, int32_t device);
+
+/* ----- runtime ----- */
+const char *gf_last_error(void);
+int32_t gf_set_kernel_timing(int32_t enable);
+/* Number of generations the drift kernel advances per launch at most
+ * (temporal blocking depth); 1 gives one generation per launch. */
+int32_t gf_set_generations_per_launch(int32_t k);
+int32_t gf_device_count(int32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOFISH_CAPI_H_ */

@@ -1,0 +1,196 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32, i64, f32 = C.c_int32, C.c_int64, C.c_float
+    sig = {
+        "orc_set_threads": (i32, [i32]),
+        "orc_philox": (None, [C.c_uint32, C.c_uint32, vp, vp]),
+        "orc_u01_array": (None, [vp, vp, i64]),
+        "orc_exp_array": (None, [vp, vp, i64]),
+        "orc_log_array": (None, [vp, vp, i64]),
+        "orc_normcdfinv_array": (None, [vp, vp, i64]),
+        "orc_poiscdfinv_array": (None, [vp, vp, vp, i64]),
+        "orc_binomcdfinv_array": (None, [vp, vp, vp, vp, vp, i64]),
+        "orc_binom_draw_array": (None, [vp, vp, vp, vp, vp, i64]),
+        "orc_pois_draw_array": (None, [vp, vp, vp, vp, i64]),
+        "orc_round_away_array": (None, [vp, vp, i64]),
+        "orc_drift_step": (i32, [i32, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+        "orc_mse_counts": (i32, [vp, i32, f32, i32, i32, vp, i32]),
+        "orc_new_count": (i32, [i32, f32, f32, i32, i32, i32, i32]),
+        "orc_run": (i32, [vp, C.POINTER(vp)]),
+        "orc_result_num_samples": (i32, [vp]),
+        "orc_result_sample_info": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint8)]),
+        "orc_result_sample_freq": (i32, [vp, i32, vp]),
+        "orc_result_num_ids": (i32, [vp]),
+        "orc_result_ids": (i32, [vp, vp]),
+        "orc_result_live_counts": (i32, [vp, vp]),
+        "orc_result_free": (None, [vp]),
+        "orc_population_histogram": (i32, [vp, i32, i32, f32, vp]),
+        "orc_site_frequency_spectrum": (i32, [vp, i32, i32, f32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n):
+    load().orc_set_threads(int(n))
+
+
+def philox(k0, k1, ctr):
+    out = np.zeros(4, np.uint32)
+    c = np.asarray(ctr, np.uint32)
+    load().orc_philox(int(k0) & 0xFFFFFFFF, int(k1) & 0xFFFFFFFF, _p(c), _p(out))
+    return out
+
+
+def _unary(name, x, out_dtype=np.float32, in_dtype=np.float32):
+    x = np.ascontiguousarray(x, in_dtype)
+    out = np.empty(x.shape, out_dtype)
+    getattr(load(), name)(_p(x), _p(out), x.size)
+    return out
+
+
+def u01(u):
+    return _unary("orc_u01_array", u, in_dtype=np.uint32)
+
+
+def exp_f(x):
+    return _unary("orc_exp_array", x)
+
+
+def log_f(x):
+    return _unary("orc_log_array", x)
+
+
+def normcdfinv(x):
+    return _unary("orc_normcdfinv_array", x)
+
+
+def round_away(x):
+    return _unary("orc_round_away_array", x)
+
+
+def poiscdfinv(r, mean):
+    r = np.ascontiguousarray(r, np.float32)
+    m = np.ascontiguousarray(np.broadcast_to(mean, r.shape), np.float32)
+    out = np.empty(r.shape, np.int32)
+    load().orc_poiscdfinv_array(_p(r), _p(m), _p(out), r.size)
+    return out
+
+
+def binomcdfinv(r, mean, x, N):
+    r = np.ascontiguousarray(r, np.float32)
+    b = lambda v: np.ascontiguousarray(np.broadcast_to(v, r.shape), np.float32)
+    m, xx, NN = b(mean), b(x), b(N)
+    out = np.empty(r.shape, np.int32)
+    load().orc_binomcdfinv_array(_p(r), _p(m), _p(xx), _p(NN), _p(out), r.size)
+    return out
+
+
+def binom_draw(bits, mean, var, N):
+    bits = np.ascontiguousarray(bits, np.uint32)
+    b = lambda v: np.ascontiguousarray(np.broadcast_to(v, bits.shape), np.float32)
+    out = np.empty(bits.shape, np.int32)
+    load().orc_binom_draw_array(_p(bits), _p(b(mean)), _p(b(var)), _p(b(N)), _p(out), bits.size)
+    return out
+
+
+def pois_draw(bits, mean, N):
+    bits = np.ascontiguousarray(bits, np.uint32)
+    b = lambda v: np.ascontiguousarray(np.broadcast_to(v, bits.shape), np.float32)
+    out = np.empty(bits.shape, np.int32)
+    load().orc_pois_draw_array(_p(bits), _p(b(mean)), _p(b(N)), _p(out), bits.size)
+    return out
+
+
+def drift_step(x, ids, g, seed1, seed2, Nc, F, h, sa, sb, mig):
+    """One generation for M mutations x[M][P] (modified copy returned)."""
+    x = np.ascontiguousarray(x, np.float32).copy()
+    M, P = x.shape
+    ids = np.ascontiguousarray(ids)
+    a = lambda v, t: np.ascontiguousarray(v, t)
+    rc = load().orc_drift_step(P, M, _p(x), _p(ids), int(g), _i32(seed1), _i32(seed2),
+                               _p(a(Nc, np.int32)), _p(a(F, np.float32)), _p(a(h, np.float32)),
+                               _p(a(sa, np.float32)), _p(a(sb, np.float32)), _p(a(mig, np.float32)))
+    assert rc == 0
+    return x
+
+
+def new_count(Nc, mu, L, seed1, seed2, g, pop):
+    return load().orc_new_count(int(Nc), float(mu), float(L), _i32(seed1), _i32(seed2), int(g), int(pop))
+
+
+def mse_counts(pg_row, pop, L, seed1, seed2):
+    """pg_row: numpy record of POP_GEN_DTYPE."""
+    rec = np.ascontiguousarray(np.array(pg_row).reshape(1))
+    Nc = int(2 * int(rec["N"][0]) / (1 + float(rec["F"][0])))
+    out = np.zeros(max(Nc, 1), np.int32)
+    n = load().orc_mse_counts(_p(rec), int(pop), float(L), _i32(seed1), _i32(seed2), _p(out), len(out))
+    return out[:n]
+
+
+def _i32(v):
+    v = int(v) & 0xFFFFFFFF
+    return v - (1 << 32) if v >= (1 << 31) else v
+
+
+def run(desc):
+    """Runs a parallelpopgen_amd.capi.SimDesc through the oracle."""
+    from parallelpopgen_amd.capi import read_result  # layout helpers only
+    lib = load()
+    h = C.c_void_p()
+    rc = lib.orc_run(C.byref(desc.desc), C.byref(h))
+    if rc != 0:
+        raise RuntimeError("oracle run failed: %d" % rc)
+    try:
+        return read_result(lib, "orc_", h, desc.P, desc.G)
+    finally:
+        lib.orc_result_free(h)
+
+
+def population_frequency_histogram(freq_row, Nc, num_sites):
+    f = np.ascontiguousarray(freq_row, np.float32)
+    out = np.empty(int(Nc), np.float32)
+    assert load().orc_population_histogram(_p(f), len(f), int(Nc), float(num_sites), _p(out)) == 0
+    return out
+
+
+def site_frequency_spectrum(freq_row, n, num_sites):
+    f = np.ascontiguousarray(freq_row, np.float32)
+    out = np.empty(int(n), np.float32)
+    assert load().orc_site_frequency_spectrum(_p(f), len(f), int(n), float(num_sites), _p(out)) == 0
+    return out

@@ -1,0 +1,272 @@
+/*
+ * gf_math.hpp -- random numbers and elementary functions shared by the
+ * gfx950 kernels and the library's host code.
+ *
+ * Every function is built from IEEE binary32 +, -, *, / and sqrt (all
+ * correctly rounded on gfx950 and on x86-64) in a fixed order; the library
+ * is compiled with -ffp-contract=off, so host and device evaluate the same
+ * operation sequence and agree bit for bit.  No device libm (OCML) or
+ * hardware approximation (v_exp_f32, v_log_f32, v_rcp_f32) is used on any
+ * value that reaches a simulation result.
+ */
+#ifndef GF_MATH_HPP_
+#define GF_MATH_HPP_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GF_HD __host__ __device__ __forceinline__
+
+namespace gf {
+
+/* ----- bit casts ----- */
+GF_HD float bits2f(uint32_t b) { return __builtin_bit_cast(float, b); }
+GF_HD uint32_t f2bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+/* ----- Philox4x32-10 -----
+ * Salmon et al., "Parallel random numbers: as easy as 1, 2, 3" (SC'11);
+ * same round function and key schedule as Random123's philox4x32 with
+ * R = 10 (the generator of reference shared.cuh:67-82). */
+struct u32x4 { uint32_t x, y, z, w; };
+
+GF_HD void mulhilo(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo) {
+    uint64_t p = (uint64_t)a * (uint64_t)b;
+    hi = (uint32_t)(p >> 32);
+    lo = (uint32_t)p;
+}
+
+GF_HD u32x4 philox10(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t hi0, lo0, hi1, lo1;
+        mulhilo(0xD2511F53u, c0, hi0, lo0);
+        mulhilo(0xCD9E8D57u, c2, hi1, lo1);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
+GF_HD uint32_t lane_of(const u32x4 &r, int i) {
+    return i == 0 ? r.x : (i == 1 ? r.y : (i == 2 ? r.z : r.w));
+}
+
+/* u * 2^-32 + 2^-33 in (0, 1] (reference shared.cuh:59-64) */
+GF_HD float u01(uint32_t u) {
+    float f = (float)u;
+    f = f * 2.3283064365386963e-10f;
+    return f + 1.1641532182693481e-10f;
+}
+
+/* ----- elementary functions ----- */
+GF_HD float pow2i(int k) { return bits2f((uint32_t)(k + 127) << 23); }
+
+GF_HD float scale2(float p, int n) {
+    if (n > 127) return (p * pow2i(127)) * pow2i(n - 127);
+    if (n < -126) return (p * pow2i(n + 100)) * pow2i(-100);
+    return p * pow2i(n);
+}
+
+/* exp(x): Cody-Waite split of ln 2 (0.693359375 + -2.12194440e-4), degree-7
+ * Taylor polynomial on |r| <= ln2/2, exact scaling by 2^n. */
+GF_HD float exp_f(float x) {
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return __builtin_inff();
+    if (x < -103.97208404541015625f) return 0.0f;
+    float t = x * 1.44269502162933349609375f;
+    float n = __builtin_floorf(t + 0.5f);
+    float r = x - n * 0.693359375f;
+    r = r - n * -2.12194440e-4f;
+    float p = 1.98412698e-4f;
+    p = p * r + 1.38888889e-3f;
+    p = p * r + 8.33333333e-3f;
+    p = p * r + 4.16666667e-2f;
+    p = p * r + 1.66666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return scale2(p, (int)n);
+}
+
+/* log(x): x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
+ * s = (m-1)/(m+1), odd series through s^9. */
+GF_HD float log_f(float x) {
+    if (x != x || x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_inff();
+    if (x == __builtin_inff()) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) { x = x * 8388608.0f; e = -23; }
+    uint32_t b = f2bits(x);
+    e += (int)((b >> 23) & 0xffu) - 127;
+    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    float q = 0.111111111f;
+    q = q * s2 + 0.142857143f;
+    q = q * s2 + 0.2f;
+    q = q * s2 + 0.333333333f;
+    float lm = (s + s) + (s + s) * (s2 * q);
+    float fe = (float)e;
+    return fe * 0.693359375f + (lm + fe * -2.12194440e-4f);
+}
+
+GF_HD float pow_f(float a, float b) {
+    if (a == 1.0f || b == 0.0f) return 1.0f;
+    if (a == 0.0f) return b > 0.0f ? 0.0f : __builtin_inff();
+    return exp_f(b * log_f(a));
+}
+
+/* inverse standard normal CDF, Wichura AS 241 PPND7 (Appl. Statist. 37,
+ * 1988, 477-484); u clamped to <= 1 - 2^-24 so the result stays finite */
+GF_HD float normcdfinv_f(float p) {
+    if (p > 0.99999994039535522f) p = 0.99999994039535522f;
+    float q = p - 0.5f;
+    if (__builtin_fabsf(q) <= 0.425f) {
+        float r = 0.180625f - q * q;
+        float num = ((5.9109374720e+01f * r + 1.5929113202e+02f) * r + 5.0434271938e+01f) * r + 3.3871327179e+00f;
+        float den = ((6.7187563600e+01f * r + 7.8757757664e+01f) * r + 1.7895169469e+01f) * r + 1.0f;
+        return q * num / den;
+    }
+    float r = q < 0.0f ? p : 1.0f - p;
+    r = __builtin_sqrtf(-log_f(r));
+    float z;
+    if (r <= 5.0f) {
+        r = r - 1.6f;
+        float num = ((1.7023821103e-01f * r + 1.3067284816e+00f) * r + 2.7568153900e+00f) * r + 1.4234372777e+00f;
+        float den = (1.2021132975e-01f * r + 7.3700164250e-01f) * r + 1.0f;
+        z = num / den;
+    } else {
+        r = r - 5.0f;
+        float num = ((1.7337203997e-02f * r + 4.2868294337e-01f) * r + 3.0812263860e+00f) * r + 6.6579046435e+00f;
+        float den = (1.2258202635e-02f * r + 2.4197894225e-01f) * r + 1.0f;
+        z = num / den;
+    }
+    return q < 0.0f ? -z : z;
+}
+
+GF_HD float round_away(float v) {
+    float t = __builtin_truncf(v);
+    float d = v - t;
+    if (__builtin_fabsf(d) >= 0.5f) t = t + (v < 0.0f ? -1.0f : 1.0f);
+    return t;
+}
+
+GF_HD int f2i_sat(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)v;
+}
+
+/* early-exit means of the truncated CDF inversions: after term j the
+ * search stops once mean <= exit_mean(j); -1 = no early exit there
+ * (reference shared.cuh:105-163 and :188-246) */
+GF_HD int exit_mean(int j) {
+    switch (j) {
+    case 11: return 1;  case 14: return 2;  case 17: return 3;  case 20: return 4;
+    case 22: return 5;  case 24: return 6;  case 26: return 7;  case 28: return 8;
+    case 30: return 9;  case 32: return 10; case 34: return 11; case 36: return 12;
+    case 38: return 13; case 40: return 14; case 41: return 15; case 43: return 16;
+    case 45: return 17; case 46: return 18; case 48: return 19; case 50: return 20;
+    case 51: return 21; case 53: return 22; case 55: return 23; case 56: return 24;
+    case 58: return 25; case 59: return 26; case 61: return 27; case 62: return 28;
+    case 64: return 29; case 65: return 30; case 67: return 31; case 68: return 32;
+    default: return -1;
+    }
+}
+
+GF_HD int poiscdfinv(float r, float mean) {
+    float emu = exp_f(-1.0f * mean);
+    float cdf = emu;
+    if (cdf >= r) return 0;
+    for (int j = 1; j < 70; j++) {
+        emu = emu * (mean / (float)j);
+        cdf = cdf + emu;
+        if (cdf >= r) return j;
+        int em = exit_mean(j);
+        if (em >= 0 && mean <= (float)em) return j;
+    }
+    return 70;
+}
+
+GF_HD int binomcdfinv(float r, float mean, float x, float n) {
+    float emu = pow_f(1.0f - x, n);
+    if (emu == 1.0f) emu = exp_f(-1.0f * mean);
+    float cdf = emu;
+    if (cdf >= r) return 0;
+    for (int j = 1; j < 70; j++) {
+        float fj = (float)j;
+        emu = emu * (((n + 1.0f - fj) * x) / (fj * (1.0f - x)));
+        cdf = cdf + emu;
+        if (cdf >= r) return j;
+        int em = exit_mean(j);
+        if (em >= 0 && mean <= (float)em) return j;
+    }
+    return 70;
+}
+
+/* approximate binomial draw (reference shared.cu:10-18) */
+GF_HD int binom_draw(uint32_t bits, float mean, float var, float N) {
+    float u = u01(bits);
+    if (mean <= 6.0f) {
+        if (N < 100.0f) return binomcdfinv(u, mean, mean / N, N);
+        return poiscdfinv(u, mean);
+    } else if (mean >= N - 6.0f) {
+        if (N < 100.0f) {
+            float nm = N - mean;
+            return f2i_sat(N - (float)binomcdfinv(u, nm, nm / N, N));
+        }
+        return f2i_sat(N - (float)poiscdfinv(u, N - mean));
+    }
+    return f2i_sat(round_away(normcdfinv_f(u) * __builtin_sqrtf(var) + mean));
+}
+
+/* approximate Poisson draw (reference shared.cuh:251-256) */
+GF_HD int pois_draw(uint32_t bits, float mean, float var, float N) {
+    float u = u01(bits);
+    if (mean <= 6.0f) return poiscdfinv(u, mean);
+    if (mean >= N - 6.0f) return f2i_sat(N - (float)poiscdfinv(u, N - mean));
+    return f2i_sat(round_away(normcdfinv_f(u) * __builtin_sqrtf(var) + mean));
+}
+
+/* ----- model helpers ----- */
+
+/* effective number of chromosomes (int)(2N/(1+F)) (reference go_fish_impl.cuh:659) */
+GF_HD int nchrom(int N, float F) {
+    float v = (float)(2 * N) / (1.0f + F);
+    return f2i_sat(__builtin_truncf(v));
+}
+
+/* equilibrium frequency weight (reference go_fish_impl.cuh:20-22) */
+GF_HD float mse_weight(float i, int N, float F, float h, float s) {
+    float A = ((2.0f * h) + ((1.0f - (2.0f * h)) * i)) * (1.0f - F) + (2.0f * F);
+    float e = ((((float)(2 * N) * s) * i) * A) / (1.0f + F);
+    return exp_f(e);
+}
+
+/* Philox counter words of the drift draw of a mutation in population pop at
+ * generation g: {|threadID|, origin_generation, origin_pop<<16 | pop, g>>2};
+ * the draw is lane g & 3. */
+GF_HD u32x4 drift_block(uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop, int pop, int g) {
+    return philox10(k0, k1, tid, (uint32_t)ogen, ((uint32_t)opop << 16) | (uint32_t)pop, (uint32_t)(g >> 2));
+}
+
+/* new-mutation count of population pop at generation g
+ * (reference calc_new_mutations_Index, go_fish_impl.cuh:454-467; counter
+ * {0, g, pop, 1}) */
+GF_HD int new_mutation_count(int Nc, float mu, float L, uint32_t k0, uint32_t k1, int g, int pop) {
+    if (Nc <= 0) return 0;
+    float lambda = (mu * (float)Nc) * L;
+    if (lambda == 0.0f) return 0;
+    u32x4 r = philox10(k0, k1, 0u, (uint32_t)g, (uint32_t)pop, 1u);
+    int n = pois_draw(r.x, lambda, lambda, (float)Nc * L);
+    return n > 0 ? n : 0;
+}
+
+} /* namespace gf */
+
+#endif /* GF_MATH_HPP_ */

@@ -1,0 +1,137 @@
+/*
+ * gf_runtime.hpp -- host-side helpers shared by the library's translation
+ * units: error reporting, device buffers, kernel timing.
+ */
+#ifndef GF_RUNTIME_HPP_
+#define GF_RUNTIME_HPP_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <chrono>
+#include <algorithm>
+#include "../../include/gofish_capi.h"
+
+namespace gf {
+
+void set_error(const std::string &msg);
+const std::string &get_error();
+bool timing_enabled();
+int generations_per_launch();
+
+struct Status {
+    int code = GF_OK;
+    explicit operator bool() const { return code != GF_OK; }
+};
+
+#define GF_HIP(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::gf::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +   \
+                            __FILE__ + ":" + std::to_string(__LINE__) + ": " #expr);       \
+            return GF_ERR_DEVICE;                                                          \
+        }                                                                                  \
+    } while (0)
+
+#define GF_TRY(expr)                     \
+    do {                                 \
+        int rc_ = (expr);                \
+        if (rc_ != GF_OK) return rc_;    \
+    } while (0)
+
+/* owning device buffer */
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        if (count <= n && p) return GF_OK;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+            return GF_ERR_ALLOC;
+        }
+        n = count;
+        return GF_OK;
+    }
+    void swap(DevBuf &o) { std::swap(p, o.p); std::swap(n, o.n); }
+};
+
+/* pinned host buffer */
+template <typename T>
+struct PinBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    PinBuf() = default;
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+    ~PinBuf() { if (p) (void)hipHostFree(p); }
+    int alloc(size_t count) {
+        if (count <= n && p) return GF_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) { p = nullptr; set_error("hipHostMalloc failed"); return GF_ERR_ALLOC; }
+        n = count;
+        return GF_OK;
+    }
+};
+
+/* HIP-event based kernel timer (on the library's stream) */
+struct KernelTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    bool on = false;
+    hipStream_t stream = nullptr;
+    void begin(hipStream_t s, hipEvent_t &a) {
+        if (!on) return;
+        (void)hipEventCreate(&a);
+        (void)hipEventRecord(a, s);
+        stream = s;
+    }
+    void end(hipEvent_t a) {
+        if (!on) return;
+        hipEvent_t b;
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(b, stream);
+        ev.emplace_back(a, b);
+    }
+    /* call after the stream has been synchronised */
+    double total_ms() {
+        double t = 0;
+        for (auto &e : ev) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) t += ms;
+        }
+        return t;
+    }
+    int64_t count() const { return (int64_t)ev.size(); }
+    ~KernelTimer() {
+        for (auto &e : ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    }
+};
+
+/* device scan of int32 counts into int64 exclusive offsets plus total */
+int scan_counts(const int *d_in, int64_t n, int64_t *d_out, int64_t *d_total, hipStream_t stream,
+                DevBuf<int64_t> &scratch);
+
+int select_device(int device);
+int num_cus();
+
+} /* namespace gf */
+
+#endif /* GF_RUNTIME_HPP_ */

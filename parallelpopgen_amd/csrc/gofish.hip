@@ -1,0 +1,645 @@
+/*
+ * gofish.hip -- host driver of one GO_Fish simulation (the plain run_sim
+ * path) and the C ABI entry points gf_run / gf_result_*.
+ *
+ * Reference: GO_Fish::run_sim, 3P/_internal/go_fish_impl.cuh:553-730.
+ * The generation loop is planned on the host once (model tables, extinction
+ * bookkeeping, parameter checks, new-mutation counts), uploaded in one copy,
+ * and then executed as blocks of up to KMAX generations per k_advance launch;
+ * a block ends at every generation that needs compaction, a time sample or
+ * a preserve flag.  The host synchronises only at compaction (survivor count)
+ * and at time samples (device-to-host copy).
+ */
+#include "gf_kernels.hpp"
+#include "gf_runtime.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+/* ---------------- result object ---------------- */
+struct gf_sample_rec {
+    int32_t num_mutations = 0, generation = 0;
+    std::vector<int32_t> Nchrom_e;
+    std::vector<uint8_t> extinct;
+    std::vector<float> freq;   /* [P][M] */
+};
+
+struct gf_result {
+    int32_t P = 0, G = 0;
+    std::vector<gf_sample_rec> samples;
+    std::vector<gf_mutid> ids;
+    std::vector<int64_t> live;
+    gf_timing timing{};
+};
+
+namespace gf {
+
+/* ---------------- runtime globals ---------------- */
+static thread_local std::string t_err;
+void set_error(const std::string &m) { t_err = m; }
+const std::string &get_error() { return t_err; }
+static bool g_timing = false;
+bool timing_enabled() { return g_timing; }
+static int g_kpl = 32;
+int generations_per_launch() { return g_kpl; }
+
+int select_device(int device) {
+    if (device >= 0) GF_HIP(hipSetDevice(device));
+    return GF_OK;
+}
+
+int num_cus() {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) return 256;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, d) != hipSuccess) return 256;
+    return pr.multiProcessorCount;
+}
+
+int scan_counts(const int *d_in, int64_t n, int64_t *d_out, int64_t *d_total, hipStream_t stream,
+                DevBuf<int64_t> &scratch) {
+    if (n <= 0) {
+        GF_HIP(hipMemsetAsync(d_total, 0, sizeof(int64_t), stream));
+        return GF_OK;
+    }
+    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (n + 4095) / 4096));
+    int64_t per = (n + nb - 1) / nb;
+    nb = (n + per - 1) / per;
+    GF_TRY(scratch.alloc((size_t)nb));
+    hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)nb), dim3(256), 0, stream, d_in, n, per, scratch.p);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, stream, scratch.p, (int)nb, d_total);
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, stream, d_in, n, per, scratch.p, d_out);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+/* ---------------- parameter checks (reference check_sim_parameters,
+ * go_fish_impl.cuh:431-452); messages match the reference text ---------------- */
+static int check_params(const gf_sim_desc *d, int t, int g, const std::vector<uint8_t> &extinct) {
+    const int P = d->num_populations;
+    const gf_pop_gen *pg = d->params + (size_t)t * P;
+    const float *mig = d->migration + (size_t)t * P * P;
+    char buf[256];
+    for (int p = 0; p < P; p++) {
+        double migration = 0;
+        if (pg[p].mu < 0) {
+            snprintf(buf, sizeof buf, "check_sim_parameters: mutation error: mu_rate < 0\tgeneration %d\t population %d", g, p);
+            set_error(buf); return GF_ERR_PARAM;
+        }
+        const int N = pg[p].N;
+        if (N > 0 && extinct[p]) {
+            snprintf(buf, sizeof buf, "check_sim_parameters: demography error: extinct population with population size > 0\tgeneration %d\t population %d", g, p);
+            set_error(buf); return GF_ERR_PARAM;
+        }
+        if (pg[p].F < 0) {
+            snprintf(buf, sizeof buf, "check_sim_parameters: inbreeding error: inbreeding coefficient < 0\tgeneration %d\t population %d", g, p);
+            set_error(buf); return GF_ERR_PARAM;
+        }
+        if (pg[p].F > 1) {
+            snprintf(buf, sizeof buf, "check_sim_parameters: inbreeding error: inbreeding coefficient > 1\tgeneration %d\t population %d", g, p);
+            set_error(buf); return GF_ERR_PARAM;
+        }
+        for (int p2 = 0; p2 < P; p2++) {
+            const float m_from = mig[p * P + p2];
+            if (m_from < 0) {
+                snprintf(buf, sizeof buf, "check_sim_parameters: migration error: migration rate < 0\tgeneration %d\t population_from %d\t population_to %d", g, p, p2);
+                set_error(buf); return GF_ERR_PARAM;
+            }
+            if (m_from > 0 && ((N <= 0 || extinct[p]) && !(pg[p2].N <= 0 || extinct[p2]))) {
+                snprintf(buf, sizeof buf, "migration error, migration from non-existant population\tgeneration %d\t population_from %d\t population_to %d", g, p, p2);
+                set_error(buf); return GF_ERR_PARAM;
+            }
+            migration += (double)mig[p2 * P + p];
+        }
+        if ((float)migration != 1.f && !(N <= 0 || extinct[p])) {
+            snprintf(buf, sizeof buf, "check_sim_parameters: migration error: migration rate does not sum to 1\tgeneration %d\t population_TO %d\t total_migration_proportion %f", g, p, migration);
+            set_error(buf); return GF_ERR_PARAM;
+        }
+    }
+    return GF_OK;
+}
+
+/* ---------------- one simulation ---------------- */
+template <int P>
+struct PlainRun {
+    const gf_sim_desc *d;
+    gf_result *R;
+    hipStream_t st = nullptr;
+    int G, start, final_g, interval;
+    uint32_t k0, k1;
+    std::vector<GenRow<P>> rows;          /* rows[t], t = 0..G */
+    std::vector<std::vector<uint8_t>> ext_at;
+    DevBuf<GenRow<P>> d_rows;
+    DevBuf<unsigned long long> d_live;
+    DevBuf<float> freqA, freqB;
+    DevBuf<gf_mutid> idsA, idsB;
+    int64_t cap = 0;
+    int M = 0;
+    /* compaction scratch */
+    DevBuf<unsigned long long> d_masks;
+    DevBuf<unsigned int> d_tcount, d_toff;
+    DevBuf<int> d_tile_seg;
+    DevBuf<int> d_small;       /* tile_first, tile_num, seg_M, seg_present, seg_preserve, seg_total */
+    DevBuf<int64_t> d_small64; /* seg_base, seg_nbase */
+    PinBuf<int> h_small;
+    PinBuf<int64_t> h_small64;
+    size_t tile_seg_n = 0;
+    KernelTimer t_adv, t_cmp;
+    int64_t slot_gens = 0;
+
+    ~PlainRun() { if (st) (void)hipStreamDestroy(st); }
+
+    int plan() {
+        const int Pn = d->num_populations;
+        (void)Pn;
+        rows.assign(G + 1, GenRow<P>{});
+        ext_at.assign(G + 1, std::vector<uint8_t>(P, 0));
+        std::vector<uint8_t> ext(P, 0);
+        if (d->init_mode == GF_INIT_PREV && d->init_num_mutations > 0 && d->init_extinct)
+            for (int p = 0; p < P; p++) ext[p] = d->init_extinct[p] ? 1 : 0;
+        std::vector<int> prevN(P);
+        int64_t cum = 0;
+        for (int t = 0; t <= G; t++) {
+            const int g = start + t;
+            GF_TRY(check_params(d, t, g, ext));
+            const gf_pop_gen *pg = d->params + (size_t)t * P;
+            const float *mig = d->migration + (size_t)t * P * P;
+            GenRow<P> &r = rows[t];
+            int Nc[P];
+            for (int p = 0; p < P; p++) {
+                if (t == 0) { Nc[p] = 0; continue; }
+                const int N = pg[p].N;
+                if (ext[p]) { Nc[p] = 0; continue; }
+                if (N <= 0) {
+                    if (prevN[p] > 0) ext[p] = 1;
+                    Nc[p] = 0;
+                    continue;
+                }
+                Nc[p] = nchrom(N, pg[p].F);
+            }
+            r.present_mask = 0;
+            r.neutral_mask = 0;
+            for (int p = 0; p < P; p++) {
+                r.Nc[p] = Nc[p];
+                r.fN[p] = (float)Nc[p];
+                const float F = pg[p].F, h = pg[p].h;
+                r.sa[p] = pg[p].sel_slope;
+                r.sb[p] = pg[p].sel_intercept;
+                r.c1[p] = (F + h) - (h * F);
+                r.c2[p] = (F + (2.0f * h)) - ((2.0f * h) * F);
+                for (int k = 0; k < P; k++) r.mig[k][p] = mig[k * P + p];
+                r.newfreq[p] = Nc[p] > 0 ? 1.0f / (float)Nc[p] : 0.0f;
+                if (pg[p].N > 0) r.present_mask |= 1 << p;
+                if (pg[p].sel_slope == 0.0f && pg[p].sel_intercept == 0.0f) r.neutral_mask |= 1 << p;
+            }
+            for (int p = 0; p < P; p++) {
+                r.newcum[p] = (int)cum;
+                if (t > 0) cum += new_mutation_count(Nc[p], pg[p].mu, d->num_sites, k0, k1, g, p);
+                if (cum > 2000000000LL) {
+                    set_error("run_sim: more than 2e9 new mutations over the run are not supported");
+                    return GF_ERR_UNSUPPORTED;
+                }
+            }
+            r.newcum[P] = (int)cum;
+            for (int p = 0; p < P; p++) prevN[p] = pg[p].N;
+            ext_at[t] = ext;
+        }
+        return GF_OK;
+    }
+
+    int ensure_capacity(int64_t need) {
+        if (need <= cap && freqA.p) return GF_OK;
+        int64_t ncap = std::max<int64_t>({need, cap + cap / 2, (int64_t)1 << 16});
+        ncap = (ncap + 255) / 256 * 256;
+        DevBuf<float> nf;
+        DevBuf<gf_mutid> ni;
+        GF_TRY(nf.alloc((size_t)P * ncap));
+        GF_TRY(ni.alloc((size_t)ncap));
+        if (M > 0) {
+            GF_HIP(hipMemcpy2DAsync(nf.p, ncap * sizeof(float), freqA.p, cap * sizeof(float),
+                                    (size_t)M * sizeof(float), P, hipMemcpyDeviceToDevice, st));
+            GF_HIP(hipMemcpyAsync(ni.p, idsA.p, (size_t)M * sizeof(gf_mutid), hipMemcpyDeviceToDevice, st));
+        }
+        GF_HIP(hipStreamSynchronize(st));
+        freqA.swap(nf);
+        idsA.swap(ni);
+        GF_TRY(freqB.alloc((size_t)P * ncap));
+        GF_TRY(idsB.alloc((size_t)ncap));
+        cap = ncap;
+        return GF_OK;
+    }
+
+    int init_mse() {
+        bool pres = d->preserve[0] || d->sample[0];
+        std::vector<MseSeg> segs;
+        int64_t coff = 0, aoff = 0, choff = 0;
+        bool any_sel = false;
+        for (int p = 0; p < P; p++) {
+            const gf_pop_gen &pg = d->params[p];
+            const int Nc = nchrom(pg.N, pg.F);
+            if (Nc <= 1) continue;
+            MseSeg s{};
+            s.Nc = Nc; s.Nind = pg.N; s.F = pg.F; s.h = pg.h; s.mu = pg.mu;
+            s.sa = pg.sel_slope; s.sb = pg.sel_intercept; s.L = d->num_sites;
+            s.k0 = k0; s.k1 = k1; s.pop = p;
+            s.neutral = (pg.sel_slope == 0.0f && pg.sel_intercept == 0.0f) ? 1 : 0;
+            s.preserve = pres ? 1 : 0;
+            s.class_off = coff; s.area_off = aoff; s.chunk_off = choff;
+            s.sim_base = 0;
+            coff += Nc - 1; aoff += Nc; choff += (Nc + MSE_CHUNK - 1) / MSE_CHUNK;
+            if (!s.neutral) any_sel = true;
+            segs.push_back(s);
+        }
+        M = 0;
+        if (segs.empty()) return ensure_capacity(1);
+        const int ns = (int)segs.size();
+        std::vector<int64_t> class_offs(ns), area_offs(ns), chunk_offs(ns);
+        for (int s = 0; s < ns; s++) { class_offs[s] = segs[s].class_off; area_offs[s] = segs[s].area_off; chunk_offs[s] = segs[s].chunk_off; }
+        DevBuf<MseSeg> d_segs;
+        DevBuf<int64_t> d_offs;
+        GF_TRY(d_segs.alloc(ns));
+        GF_TRY(d_offs.alloc(3 * (size_t)ns));
+        GF_HIP(hipMemcpyAsync(d_segs.p, segs.data(), ns * sizeof(MseSeg), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_offs.p, class_offs.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_offs.p + ns, area_offs.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_offs.p + 2 * ns, chunk_offs.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        DevBuf<float> d_area, d_tot;
+        const unsigned grid = (unsigned)std::max(1, num_cus() * 8);
+        if (any_sel) {
+            GF_TRY(d_area.alloc((size_t)aoff));
+            GF_TRY(d_tot.alloc((size_t)choff));
+            hipLaunchKernelGGL(k_mse_area, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + ns, ns, aoff, d_area.p);
+            hipLaunchKernelGGL(k_mse_chunk_local, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + 2 * ns, ns, choff, d_area.p, d_tot.p);
+            hipLaunchKernelGGL(k_mse_chunk_prefix, dim3((ns + 63) / 64), dim3(64), 0, st, d_segs.p, ns, d_tot.p);
+            hipLaunchKernelGGL(k_mse_chunk_add, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + ns, ns, aoff, d_area.p, d_tot.p);
+            GF_HIP(hipGetLastError());
+        }
+        DevBuf<int> d_counts;
+        DevBuf<int64_t> d_scan, d_scratch, d_total;
+        GF_TRY(d_counts.alloc((size_t)coff));
+        GF_TRY(d_scan.alloc((size_t)coff));
+        GF_TRY(d_total.alloc(1));
+        hipLaunchKernelGGL(k_mse_counts, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p, ns, coff,
+                           any_sel ? d_area.p : nullptr, d_counts.p);
+        GF_HIP(hipGetLastError());
+        GF_TRY(scan_counts(d_counts.p, coff, d_scan.p, d_total.p, st, d_scratch));
+        /* per-segment first offsets and the total */
+        std::vector<int64_t> seg_first(ns);
+        for (int s = 0; s < ns; s++)
+            GF_HIP(hipMemcpyAsync(&seg_first[s], d_scan.p + segs[s].class_off, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        int64_t total = 0;
+        GF_HIP(hipMemcpyAsync(&total, d_total.p, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        if (total > 2000000000LL) { set_error("initial mutation count exceeds 2e9"); return GF_ERR_UNSUPPORTED; }
+        for (int s = 0; s < ns; s++) segs[s].slot_base = seg_first[s];
+        M = 0;
+        GF_TRY(ensure_capacity(total + 1));
+        GF_HIP(hipMemcpyAsync(d_segs.p, segs.data(), ns * sizeof(MseSeg), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_offs.p, seg_first.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        if (total > 0) {
+            /* zero every row first: k_mse_fill writes all P rows of each slot */
+            hipLaunchKernelGGL(k_mse_fill<P>, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p, ns, total,
+                               d_scan.p, freqA.p, (int64_t)cap, idsA.p);
+            GF_HIP(hipGetLastError());
+        }
+        GF_HIP(hipStreamSynchronize(st));
+        M = (int)total;
+        return GF_OK;
+    }
+
+    int init_prev() {
+        const int n = d->init_num_mutations;
+        M = 0;
+        GF_TRY(ensure_capacity(std::max(n, 1)));
+        if (n > 0) {
+            if (!d->init_freq || !d->init_ids) { set_error("init_freq/init_ids missing"); return GF_ERR_ARG; }
+            const bool pres = d->preserve[0] || d->sample[0];
+            std::vector<gf_mutid> ids(d->init_ids, d->init_ids + n);
+            if (pres)
+                for (auto &m : ids) { int z = m.origin_threadID; m.origin_threadID = -(z < 0 ? -z : z); }
+            GF_HIP(hipMemcpy2DAsync(freqA.p, cap * sizeof(float), d->init_freq, (size_t)n * sizeof(float),
+                                    (size_t)n * sizeof(float), P, hipMemcpyHostToDevice, st));
+            GF_HIP(hipMemcpyAsync(idsA.p, ids.data(), (size_t)n * sizeof(gf_mutid), hipMemcpyHostToDevice, st));
+            GF_HIP(hipStreamSynchronize(st));
+        }
+        M = n;
+        return GF_OK;
+    }
+
+    int ensure_tiles(int64_t tiles) {
+        if ((size_t)tiles <= tile_seg_n && d_tile_seg.p) return GF_OK;
+        size_t n = std::max<size_t>((size_t)tiles, 1024);
+        GF_TRY(d_tile_seg.alloc(n));
+        GF_HIP(hipMemsetAsync(d_tile_seg.p, 0, n * sizeof(int), st));
+        GF_TRY(d_masks.alloc(n * (WG / 64)));
+        GF_TRY(d_tcount.alloc(n));
+        GF_TRY(d_toff.alloc(n));
+        tile_seg_n = n;
+        return GF_OK;
+    }
+
+    /* count of slots the compaction rule keeps (initial live count) or the
+     * compaction itself (reference compact, go_fish_impl.cuh:370-429) */
+    int flag_scan(int present_mask, bool preserve, int &survivors) {
+        const int64_t tiles = (M + WG - 1) / WG;
+        GF_TRY(ensure_tiles(tiles));
+        GF_TRY(h_small.alloc(16));
+        GF_TRY(h_small64.alloc(4));
+        GF_TRY(d_small.alloc(16));
+        GF_TRY(d_small64.alloc(4));
+        int *hs = h_small.p;
+        hs[0] = 0;                  /* tile_first */
+        hs[1] = (int)tiles;         /* tile_num */
+        hs[2] = M;                  /* seg_M */
+        hs[3] = present_mask;       /* seg_present */
+        hs[4] = preserve ? 1 : 0;   /* seg_preserve */
+        hs[5] = 0;                  /* seg_total (out) */
+        h_small64.p[0] = 0;         /* seg_base */
+        h_small64.p[1] = 0;         /* seg_nbase */
+        GF_HIP(hipMemcpyAsync(d_small.p, hs, 6 * sizeof(int), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_small64.p, h_small64.p, 2 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_flag<P>, dim3((unsigned)tiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
+                           d_small64.p, d_small.p + 2, d_small.p + 3, d_tile_seg.p, d_small.p + 0,
+                           d_masks.p, d_tcount.p);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, d_tcount.p, d_small.p + 0, d_small.p + 1,
+                           d_toff.p, d_small.p + 5);
+        GF_HIP(hipGetLastError());
+        GF_HIP(hipMemcpyAsync(&hs[8], d_small.p + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        survivors = hs[8];
+        return GF_OK;
+    }
+
+    int compact(int present_mask, bool preserve) {
+        if (M == 0) return GF_OK;
+        hipEvent_t ev = nullptr;
+        t_cmp.begin(st, ev);
+        int surv = 0;
+        GF_TRY(flag_scan(present_mask, preserve, surv));
+        const int64_t tiles = (M + WG - 1) / WG;
+        hipLaunchKernelGGL(k_scatter<P>, dim3((unsigned)tiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
+                           freqB.p, (int64_t)cap, idsB.p, d_small64.p, d_small64.p + 1, d_small.p + 4,
+                           d_tile_seg.p, d_small.p + 0, d_masks.p, d_toff.p);
+        GF_HIP(hipGetLastError());
+        t_cmp.end(ev);
+        freqA.swap(freqB);
+        idsA.swap(idsB);
+        M = surv;
+        return GF_OK;
+    }
+
+    int take_sample(int t, bool with_ids) {
+        gf_sample_rec s;
+        s.num_mutations = M;
+        s.generation = start + t;
+        s.freq.resize((size_t)P * M);
+        if (M > 0)
+            GF_HIP(hipMemcpy2DAsync(s.freq.data(), (size_t)M * sizeof(float), freqA.p, cap * sizeof(float),
+                                    (size_t)M * sizeof(float), P, hipMemcpyDeviceToHost, st));
+        if (with_ids) {
+            R->ids.resize(M);
+            if (M > 0) GF_HIP(hipMemcpyAsync(R->ids.data(), idsA.p, (size_t)M * sizeof(gf_mutid), hipMemcpyDeviceToHost, st));
+        }
+        GF_HIP(hipStreamSynchronize(st));
+        const gf_pop_gen *pg = d->params + (size_t)t * P;
+        for (int p = 0; p < P; p++) {
+            s.Nchrom_e.push_back(nchrom(pg[p].N, pg[p].F));
+            s.extinct.push_back(ext_at[t][p]);
+        }
+        R->samples.push_back(std::move(s));
+        return GF_OK;
+    }
+
+    int launch_block(int t_first, int K) {
+        /* rows t_first .. t_first+K-1 ; generation before the block = start + t_first - 1 */
+        const int64_t new_in_block = (int64_t)rows[t_first + K - 1].newcum[P] - rows[t_first].newcum[0];
+        const int64_t Mend = (int64_t)M + new_in_block;
+        if (Mend > 2000000000LL) { set_error("more than 2e9 mutation slots are not supported"); return GF_ERR_UNSUPPORTED; }
+        GF_TRY(ensure_capacity(Mend));
+        if (Mend > 0) {
+            SegLaunch<P> sl{};
+            sl.rows = d_rows.p + t_first;
+            sl.live = d_live.p;
+            sl.base = 0;
+            sl.M0 = M;
+            sl.Mend = (int)Mend;
+            sl.cum_base = rows[t_first].newcum[0];
+            sl.K = K;
+            sl.g0 = start + t_first - 1;
+            sl.t0 = t_first;
+            sl.k0 = k0; sl.k1 = k1;
+            const int tiles = (int)((Mend + WG - 1) / WG);
+            const int grid = std::min(tiles, num_cus() * 8);
+            hipEvent_t ev = nullptr;
+            t_adv.begin(st, ev);
+            hipLaunchKernelGGL((k_advance<P, false>), dim3(grid), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
+                               sl, (const SegLaunch<P> *)nullptr, (const int *)nullptr, (const int *)nullptr, tiles);
+            GF_HIP(hipGetLastError());
+            t_adv.end(ev);
+            slot_gens += (int64_t)M * K + new_in_block * K / 2;
+        }
+        M = (int)Mend;
+        return GF_OK;
+    }
+
+    int run() {
+        G = d->num_generations;
+        start = d->init_mode == GF_INIT_PREV ? d->start_generation : 0;
+        final_g = start + G;
+        interval = d->compact_interval;
+        k0 = (uint32_t)d->seed1;
+        k1 = (uint32_t)d->seed2;
+        R->P = P;
+        R->G = G;
+        t_adv.on = t_cmp.on = timing_enabled();
+        auto t_start = std::chrono::steady_clock::now();
+        GF_TRY(select_device(d->device));
+        GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        GF_TRY(plan());
+        GF_TRY(d_rows.alloc(rows.size()));
+        GF_HIP(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(GenRow<P>), hipMemcpyHostToDevice, st));
+        GF_TRY(d_live.alloc((size_t)(G + 1) * 64));
+        GF_HIP(hipMemsetAsync(d_live.p, 0, (size_t)(G + 1) * 64 * sizeof(unsigned long long), st));
+
+        hipEvent_t i0, i1;
+        GF_HIP(hipEventCreate(&i0));
+        GF_HIP(hipEventCreate(&i1));
+        GF_HIP(hipEventRecord(i0, st));
+        if (d->init_mode == GF_INIT_MSE) GF_TRY(init_mse());
+        else if (d->init_mode == GF_INIT_PREV) GF_TRY(init_prev());
+        else { M = 0; GF_TRY(ensure_capacity(1)); }
+        GF_HIP(hipEventRecord(i1, st));
+
+        /* live count of the initial state */
+        int64_t live0 = 0;
+        if (M > 0) {
+            int present = 0;
+            for (int p = 0; p < P; p++) if (d->params[p].N > 0) present |= 1 << p;
+            int surv = 0;
+            GF_TRY(flag_scan(present, false, surv));
+            live0 = surv;
+        }
+        if (d->sample[0] && start != final_g) GF_TRY(take_sample(0, false));
+
+        int next_compact = start + interval;
+        int t = 0;   /* generations done */
+        const int kpl = std::max(1, std::min(KMAX, generations_per_launch()));
+        while (t < G) {
+            int K = 0;
+            int tb = t + 1;
+            for (;;) {
+                K++;
+                const int te = t + K;
+                const int g = start + te;
+                const bool pres = d->preserve[te] || d->sample[te];
+                const bool cmp = interval > 0 && (g == next_compact || g == final_g || pres);
+                if (cmp || d->sample[te] || te == G || K == kpl) break;
+            }
+            GF_TRY(launch_block(tb, K));
+            t += K;
+            const int g = start + t;
+            const bool pres = d->preserve[t] || d->sample[t];
+            if ((g == next_compact || g == final_g || pres) && interval > 0) {
+                GF_TRY(compact(rows[t].present_mask, pres));
+                next_compact = g + interval;
+            }
+            if (d->sample[t] && g != final_g) GF_TRY(take_sample(t, false));
+        }
+        GF_TRY(take_sample(G, true));
+        GF_HIP(hipStreamSynchronize(st));
+
+        /* live counts */
+        std::vector<unsigned long long> lv((size_t)(G + 1) * 64);
+        GF_HIP(hipMemcpy(lv.data(), d_live.p, lv.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        R->live.assign(G + 1, 0);
+        R->live[0] = live0;
+        for (int tt = 1; tt <= G; tt++) {
+            int64_t s = 0;
+            for (int k = 0; k < 64; k++) s += (int64_t)lv[(size_t)tt * 64 + k];
+            R->live[tt] = s;
+        }
+        float ims = 0;
+        (void)hipEventElapsedTime(&ims, i0, i1);
+        (void)hipEventDestroy(i0);
+        (void)hipEventDestroy(i1);
+        R->timing.init_ms = ims;
+        R->timing.advance_ms = t_adv.total_ms();
+        R->timing.advance_launches = t_adv.count();
+        R->timing.compact_ms = t_cmp.total_ms();
+        R->timing.compact_launches = t_cmp.count();
+        R->timing.advance_slot_generations = slot_gens;
+        R->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        return GF_OK;
+    }
+};
+
+template <int P>
+static int run_p(const gf_sim_desc *d, gf_result *R) {
+    PlainRun<P> r;
+    r.d = d;
+    r.R = R;
+    return r.run();
+}
+
+} /* namespace gf */
+
+/* ---------------- C ABI ---------------- */
+extern "C" {
+
+const char *gf_last_error(void) { return gf::get_error().c_str(); }
+
+int32_t gf_set_kernel_timing(int32_t enable) { gf::g_timing = enable != 0; return GF_OK; }
+
+int32_t gf_set_generations_per_launch(int32_t k) {
+    if (k < 1 || k > gf::KMAX) { gf::set_error("generations per launch must be in [1, 64]"); return GF_ERR_ARG; }
+    gf::g_kpl = k;
+    return GF_OK;
+}
+
+int32_t gf_device_count(int32_t *out) {
+    int n = 0;
+    GF_HIP(hipGetDeviceCount(&n));
+    if (out) *out = n;
+    return GF_OK;
+}
+
+int32_t gf_run(const gf_sim_desc *d, gf_result **out) {
+    if (!out) { gf::set_error("gf_run: out is NULL"); return GF_ERR_ARG; }
+    *out = nullptr;
+    if (!d || !d->params || !d->migration || !d->preserve || !d->sample) {
+        gf::set_error("gf_run: descriptor or one of its tables is NULL");
+        return GF_ERR_ARG;
+    }
+    if (d->num_populations < 1 || d->num_populations > GF_MAX_POPULATIONS) {
+        gf::set_error("gf_run: num_populations must be in [1, GF_MAX_POPULATIONS]");
+        return GF_ERR_ARG;
+    }
+    if (d->num_generations < 0 || d->init_mode < 0 || d->init_mode > 2 || d->compact_interval < 0 ||
+        (d->init_mode == GF_INIT_PREV && d->init_num_mutations < 0)) {
+        gf::set_error("gf_run: invalid num_generations, init_mode, compact_interval or init_num_mutations");
+        return GF_ERR_ARG;
+    }
+    std::unique_ptr<gf_result> R(new gf_result);
+    int rc;
+    switch (d->num_populations) {
+    case 1: rc = gf::run_p<1>(d, R.get()); break;
+    case 2: rc = gf::run_p<2>(d, R.get()); break;
+    case 3: rc = gf::run_p<3>(d, R.get()); break;
+    case 4: rc = gf::run_p<4>(d, R.get()); break;
+    case 5: rc = gf::run_p<5>(d, R.get()); break;
+    case 6: rc = gf::run_p<6>(d, R.get()); break;
+    case 7: rc = gf::run_p<7>(d, R.get()); break;
+    default: rc = gf::run_p<8>(d, R.get()); break;
+    }
+    if (rc != GF_OK) return rc;
+    *out = R.release();
+    return GF_OK;
+}
+
+int32_t gf_result_num_samples(const gf_result *r) { return r ? (int32_t)r->samples.size() : GF_ERR_ARG; }
+
+int32_t gf_result_sample_info(const gf_result *r, int32_t s, int32_t *M, int32_t *gen, int32_t *Nc, uint8_t *ext) {
+    if (!r || s < 0 || s >= (int32_t)r->samples.size()) { gf::set_error("sample index out of range"); return GF_ERR_ARG; }
+    const gf_sample_rec &x = r->samples[s];
+    if (M) *M = x.num_mutations;
+    if (gen) *gen = x.generation;
+    for (int p = 0; p < r->P; p++) {
+        if (Nc) Nc[p] = x.Nchrom_e[p];
+        if (ext) ext[p] = x.extinct[p];
+    }
+    return GF_OK;
+}
+
+int32_t gf_result_sample_freq(const gf_result *r, int32_t s, float *out) {
+    if (!r || !out || s < 0 || s >= (int32_t)r->samples.size()) { gf::set_error("sample index out of range"); return GF_ERR_ARG; }
+    const gf_sample_rec &x = r->samples[s];
+    if (!x.freq.empty()) memcpy(out, x.freq.data(), x.freq.size() * sizeof(float));
+    return GF_OK;
+}
+
+int32_t gf_result_num_ids(const gf_result *r) { return r ? (int32_t)r->ids.size() : GF_ERR_ARG; }
+
+int32_t gf_result_ids(const gf_result *r, gf_mutid *out) {
+    if (!r || !out) return GF_ERR_ARG;
+    if (!r->ids.empty()) memcpy(out, r->ids.data(), r->ids.size() * sizeof(gf_mutid));
+    return GF_OK;
+}
+
+int32_t gf_result_live_counts(const gf_result *r, int64_t *out) {
+    if (!r || !out) return GF_ERR_ARG;
+    memcpy(out, r->live.data(), r->live.size() * sizeof(int64_t));
+    return GF_OK;
+}
+
+int32_t gf_result_timing(const gf_result *r, gf_timing *out) {
+    if (!r || !out) return GF_ERR_ARG;
+    *out = r->timing;
+    return GF_OK;
+}
+
+void gf_result_free(gf_result *r) { delete r; }
+
+} /* extern "C" */

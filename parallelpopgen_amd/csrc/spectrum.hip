@@ -1,0 +1,151 @@
+/*
+ * spectrum.hip -- Spectrum API kernels and C ABI:
+ *   gf_population_histogram     (reference spectrum.cu:153-210)
+ *   gf_site_frequency_spectrum  (reference spectrum.cu:213-294)
+ *
+ * Both results are integer (or 2^-32 fixed-point) sums, so they do not depend
+ * on thread or workgroup order: the reference's float atomics
+ * (spectrum.cu:134-138) are replaced by 64-bit integer LDS/global atomics.
+ */
+#include "gf_spectrum.hpp"
+#include "gf_runtime.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace gf {
+
+void log_binom_table(int n, std::vector<float> &lc, int &half_n) {
+    half_n = (n % 2 == 0) ? n / 2 + 1 : (n + 1) / 2;
+    lc.assign(half_n, 0.0f);
+    float acc = 0.0f;
+    for (int k = 1; k < half_n; k++) {
+        acc = acc + log_f(((float)n + 1.0f - (float)k) / (float)k);
+        lc[k] = acc;
+    }
+}
+
+void finish_sfs(const unsigned long long *acc, unsigned long long nonseg, int n, int M, float num_sites, float *out) {
+    const double sc = 1.0 / 4294967296.0;
+    const double b0 = (double)(acc[0] + acc[n]) * sc + (double)nonseg + ((double)num_sites - (double)M);
+    out[0] = (float)b0;
+    for (int k = 1; k < n; k++) out[k] = (float)((double)acc[k] * sc);
+}
+
+int sfs_device(const float *d_freq, const int64_t *d_seg_base, const int *d_seg_M, const int *d_tile_seg,
+               const int *d_tile_first, int num_tiles, int nseg, int n, unsigned long long *d_acc,
+               hipStream_t st, DevBuf<float> &d_lc) {
+    std::vector<float> lc;
+    int half_n;
+    log_binom_table(n, lc, half_n);
+    GF_TRY(d_lc.alloc(lc.size()));
+    GF_HIP(hipMemcpyAsync(d_lc.p, lc.data(), lc.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemsetAsync(d_acc, 0, (size_t)nseg * (n + 2) * sizeof(unsigned long long), st));
+    if (num_tiles <= 0) return GF_OK;
+    const bool lds = (n + 2) <= SFS_LDS_BINS;
+    if (lds)
+        hipLaunchKernelGGL((k_sfs<true>), dim3(num_tiles), dim3(WG), 0, st, d_freq, d_seg_base, d_seg_M,
+                           d_tile_seg, d_tile_first, n, half_n, d_lc.p, d_acc);
+    else
+        hipLaunchKernelGGL((k_sfs<false>), dim3(num_tiles), dim3(WG), 0, st, d_freq, d_seg_base, d_seg_M,
+                           d_tile_seg, d_tile_first, n, half_n, d_lc.p, d_acc);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+} /* namespace gf */
+
+using namespace gf;
+
+extern "C" {
+
+int32_t gf_population_histogram(const float *freq, int32_t M, int32_t Nc, float num_sites, float *out, int32_t device) {
+    if (!out || Nc <= 0 || M < 0 || (M > 0 && !freq)) {
+        set_error("population_frequency_histogram: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    GF_TRY(select_device(device));
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = GF_OK;
+    {
+        DevBuf<float> d_f;
+        DevBuf<unsigned int> d_h;
+        rc = d_f.alloc(std::max(M, 1));
+        if (!rc) rc = d_h.alloc(Nc);
+        if (!rc) {
+            std::vector<unsigned int> h(Nc);
+            hipError_t e = hipSuccess;
+            if (M > 0) e = hipMemcpyAsync(d_f.p, freq, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemsetAsync(d_h.p, 0, (size_t)Nc * sizeof(unsigned int), st);
+            if (e == hipSuccess && M > 0) {
+                const int grid = std::min((M + WG - 1) / WG, num_cus() * 4);
+                if (Nc <= HIST_LDS_BINS)
+                    hipLaunchKernelGGL((k_pop_hist<true>), dim3(grid), dim3(WG), 0, st, d_f.p, M, Nc, d_h.p);
+                else
+                    hipLaunchKernelGGL((k_pop_hist<false>), dim3(grid), dim3(WG), 0, st, d_f.p, M, Nc, d_h.p);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_h.p, (size_t)Nc * sizeof(unsigned int), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                set_error(std::string("population_frequency_histogram: ") + hipGetErrorString(e));
+                rc = GF_ERR_DEVICE;
+            } else {
+                /* bin 0 also holds the monomorphic sites (spectrum.cu:82) */
+                const int ns = f2i_sat(__builtin_truncf(num_sites));
+                h[0] += (unsigned int)(ns - M);
+                for (int i = 0; i < Nc; i++) out[i] = (float)h[i];
+            }
+        }
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int32_t gf_site_frequency_spectrum(const float *freq, int32_t M, int32_t n, float num_sites, float *out, int32_t device) {
+    if (!out || n <= 0 || M < 0 || (M > 0 && !freq)) {
+        set_error("site_frequency_spectrum: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    GF_TRY(select_device(device));
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = GF_OK;
+    {
+        DevBuf<float> d_f, d_lc;
+        DevBuf<unsigned long long> d_acc;
+        DevBuf<int> d_meta;
+        DevBuf<int64_t> d_base;
+        const int tiles = (M + WG - 1) / WG;
+        rc = d_f.alloc(std::max(M, 1));
+        if (!rc) rc = d_acc.alloc((size_t)n + 2);
+        if (!rc) rc = d_meta.alloc((size_t)tiles + 2);
+        if (!rc) rc = d_base.alloc(1);
+        if (!rc) {
+            std::vector<int> meta((size_t)tiles + 2, 0);
+            meta[tiles] = M;      /* seg_M */
+            meta[tiles + 1] = 0;  /* tile_first */
+            const int64_t base0 = 0;
+            hipError_t e = hipSuccess;
+            if (M > 0) e = hipMemcpyAsync(d_f.p, freq, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_meta.p, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_base.p, &base0, sizeof(int64_t), hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = GF_ERR_DEVICE; }
+            if (!rc) rc = sfs_device(d_f.p, d_base.p, d_meta.p + tiles, d_meta.p, d_meta.p + tiles + 1, tiles, 1, n,
+                                     d_acc.p, st, d_lc);
+            std::vector<unsigned long long> acc((size_t)n + 2);
+            if (!rc) {
+                e = hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = GF_ERR_DEVICE; }
+            }
+            if (!rc) finish_sfs(acc.data(), acc[n + 1], n, M, num_sites, out);
+        }
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+} /* extern "C" */

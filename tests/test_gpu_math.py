@@ -1,0 +1,69 @@
+"""Device random-number / elementary functions vs the oracle, bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_philox_device_matches_reference_vectors(gpu):
+    from tests.test_oracle_math import load_kat
+    kat = load_kat()
+    out = gpu.device_math("philox", kat[:, :6].astype(np.uint32))
+    assert np.array_equal(out, kat[:, 6:].astype(np.uint32))
+
+
+def test_philox_device_matches_oracle_random(gpu, oracle):
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, 2**32, (4096, 6), dtype=np.uint64).astype(np.uint32)
+    out = gpu.device_math("philox", a)
+    ref = np.array([oracle.philox(r[0], r[1], r[2:]) for r in a])
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("op", ["exp", "log", "normcdfinv", "sqrt", "round_away"])
+def test_unary_bit_exact(gpu, oracle, op):
+    rng = np.random.default_rng(12)
+    if op == "exp":
+        x = np.concatenate([rng.uniform(-105, 90, 1 << 20), rng.uniform(-1, 1, 1 << 16)])
+    elif op == "log":
+        x = np.concatenate([np.exp(rng.uniform(-103, 88, 1 << 20)), [1e-42, 1e-45, 0.0, 1.0]])
+    elif op == "normcdfinv":
+        x = np.concatenate([rng.uniform(0, 1, 1 << 20), np.exp(rng.uniform(-23, -0.1, 1 << 16)), [1.0, 2.0**-33]])
+    elif op == "sqrt":
+        x = np.concatenate([rng.uniform(0, 1e6, 1 << 20), np.exp(rng.uniform(-100, 80, 1 << 16))])
+    else:
+        x = np.concatenate([rng.uniform(-1e6, 1e6, 1 << 18), np.arange(-1000, 1000) * 0.5])
+    x = x.astype(np.float32)
+    got = gpu.device_math(op, x)
+    if op == "sqrt":
+        ref = np.sqrt(x)
+    else:
+        ref = getattr(oracle, {"exp": "exp_f", "log": "log_f", "normcdfinv": "normcdfinv",
+                               "round_away": "round_away"}[op])(x)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), op
+
+
+def test_u01_and_div_bit_exact(gpu, oracle):
+    rng = np.random.default_rng(13)
+    u = rng.integers(0, 2**32, 1 << 20, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(gpu.device_math("u01", u), oracle.u01(u))
+    a = rng.uniform(-1e5, 1e5, 1 << 20).astype(np.float32)
+    b = rng.uniform(1e-3, 1e5, 1 << 20).astype(np.float32)
+    assert np.array_equal(gpu.device_math("div", a, b).view(np.uint32), (a / b).view(np.uint32))
+
+
+def test_binomial_and_poisson_draws_bit_exact(gpu, oracle):
+    rng = np.random.default_rng(14)
+    n = 1 << 20
+    bits = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    N = np.where(rng.random(n) < 0.2, rng.integers(2, 100, n), rng.integers(100, 400000, n)).astype(np.float32)
+    x = np.where(rng.random(n) < 0.5, rng.random(n) * 10 / N, rng.random(n)).astype(np.float32)
+    x[::97] = 1.0
+    x[::89] = 0.0
+    mean = (x * N).astype(np.float32)
+    var = ((np.float32(1) - x) * mean).astype(np.float32)
+    got = gpu.device_math("binom_draw", bits, mean, var, N)
+    assert np.array_equal(got, oracle.binom_draw(bits, mean, var, N))
+    lam = np.exp(rng.uniform(-5, 14, n)).astype(np.float32)
+    NN = (lam * 1e4).astype(np.float32)
+    assert np.array_equal(gpu.device_math("pois_draw", bits, lam, NN), oracle.pois_draw(bits, lam, NN))

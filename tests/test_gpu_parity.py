@@ -1,0 +1,152 @@
+"""GPU run_sim path vs the CPU oracle on the same tables: every time sample's
+frequencies, the final mutation IDs and the per-generation live counts must
+be identical bit for bit."""
+import numpy as np
+import pytest
+
+from parallelpopgen_amd import scenarios as S
+from parallelpopgen_amd.capi import GF_INIT_BLANK, GF_INIT_PREV, POP_GEN_DTYPE, SimDesc
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b, what=""):
+    assert len(a["samples"]) == len(b["samples"]), what
+    for i, (x, y) in enumerate(zip(a["samples"], b["samples"])):
+        assert x["generation"] == y["generation"], (what, i)
+        assert x["num_mutations"] == y["num_mutations"], (what, i, x["num_mutations"], y["num_mutations"])
+        assert x["Nchrom_e"] == y["Nchrom_e"] and x["extinct"] == y["extinct"], (what, i)
+        fx, fy = x["freq"].view(np.uint32), y["freq"].view(np.uint32)
+        if not np.array_equal(fx, fy):
+            bad = np.argwhere(fx != fy)
+            raise AssertionError("%s sample %d: %d mismatches, first %s gpu=%r oracle=%r" % (
+                what, i, len(bad), bad[0], x["freq"][tuple(bad[0])], y["freq"][tuple(bad[0])]))
+    assert np.array_equal(a["ids"], b["ids"]), what
+    assert np.array_equal(a["live"], b["live"]), (what, np.argwhere(a["live"] != b["live"])[:5])
+
+
+def test_plain_small_full(gpu, oracle):
+    d = S.plain_small()
+    g = gpu.run(d)
+    o = oracle.run(d)
+    assert_same(g, o, "plain_small")
+    assert g["samples"][-1]["num_mutations"] > 7000
+
+
+@pytest.mark.parametrize("kpl", [1, 5, 64])
+def test_generations_per_launch_invariance(gpu, oracle, kpl):
+    d = S.plain_small(G=150, compact_interval=20)
+    ref = oracle.run(d)
+    gpu.set_generations_per_launch(kpl)
+    try:
+        g = gpu.run(d)
+    finally:
+        gpu.set_generations_per_launch(32)
+    assert_same(g, ref, "kpl=%d" % kpl)
+
+
+@pytest.mark.parametrize("interval", [0, 1, 7, 35])
+def test_compaction_interval(gpu, oracle, interval):
+    d = S.plain_small(G=120, compact_interval=interval)
+    assert_same(gpu.run(d), oracle.run(d), "interval=%d" % interval)
+
+
+def test_compaction_does_not_change_single_population_results(gpu):
+    a = gpu.run(S.plain_small(G=200, compact_interval=10))
+    b = gpu.run(S.plain_small(G=200, compact_interval=45))
+    assert np.array_equal(a["samples"][-1]["freq"], b["samples"][-1]["freq"])
+    assert np.array_equal(a["ids"], b["ids"])
+
+
+def test_plain_large_selected_scaled(gpu, oracle):
+    # gamma = -10, h = 0.5 at N = 1e5 with L scaled down so the oracle is quick
+    d = S.plain_large(L=2e6, G=300)
+    assert_same(gpu.run(d), oracle.run(d), "plain_large L=2e6")
+
+
+def test_three_population_scaled(gpu, oracle):
+    d = S.three_population(scale=0.1, L=2e8, G=700, preserve_every=100, sample_every=200)
+    g, o = gpu.run(d), oracle.run(d)
+    assert_same(g, o, "three_population")
+    assert len(g["samples"]) == 4
+
+
+def test_blank_start_small_N_binomial(gpu, oracle):
+    # N < 100 chromosomes exercises the binomial CDF inversion
+    d = S.constant(N=30, L=1e9, G=400, mu=1e-8, F=0.0, h=0.2, s=-0.02, init_mode=GF_INIT_BLANK,
+                   compact_interval=13)
+    assert_same(gpu.run(d), oracle.run(d), "blank N=30")
+
+
+def test_positive_selection_dominance_inbreeding(gpu, oracle):
+    d = S.constant(N=5000, L=5e7, G=300, F=0.3, h=1.0, s=0.004, compact_interval=9)
+    assert_same(gpu.run(d), oracle.run(d), "s>0")
+    d = S.constant(N=5000, L=5e7, G=300, F=1.0, h=0.0, s=-0.01, compact_interval=9)
+    assert_same(gpu.run(d), oracle.run(d), "haploid")
+
+
+def split_scenario(G=400, L=5e7):
+    """population 1 splits off population 0 at generation 100, grows, then
+    both exchange migrants; population 0 goes extinct at generation 300."""
+    P = 2
+    params = np.zeros((G + 1, P), POP_GEN_DTYPE)
+    g = np.arange(G + 1)
+    params["N"][:, 0] = np.where(g < 300, 8000, 0)
+    params["N"][:, 1] = np.where(g < 100, 0, 2000 + 10 * (g - 100))
+    params["F"] = 0.0
+    params["h"] = 0.5
+    params["mu"] = 2e-9
+    params["sel_slope"][:, 1] = -0.002
+    params["sel_intercept"][:, 1] = 0.001
+    mig = np.zeros((G + 1, P, P), np.float32)
+    mig[:, 0, 0] = 1
+    mig[:, 1, 1] = 1
+    mig[100, 0, 1] = 1.0   # at the split pop 1 is made of pop 0 chromosomes
+    mig[100, 1, 1] = 0.0
+    m = np.float32(0.01)
+    for t in range(101, 300):
+        mig[t] = [[1 - m, m], [m, 1 - m]]
+    sample = np.zeros(G + 1, np.uint8)
+    sample[[150, 299]] = 1
+    return SimDesc(P, L, 0x1234, 0x5678, G, params, mig, None, sample, 25)
+
+
+def test_split_migration_extinction(gpu, oracle):
+    d = split_scenario()
+    g, o = gpu.run(d), oracle.run(d)
+    assert_same(g, o, "split")
+    assert g["samples"][-1]["extinct"] == [True, False]
+
+
+def test_prev_sim_restart(gpu, oracle):
+    a = gpu.run(S.plain_small(G=100, compact_interval=10))
+    last = a["samples"][-1]
+    G = 80
+    params = np.zeros((G + 1, 1), POP_GEN_DTYPE)
+    params["N"] = 10000
+    params["h"] = 0.5
+    params["mu"] = 1e-9
+    params["sel_intercept"] = -1e-4
+    mig = np.ones((G + 1, 1, 1), np.float32)
+    sample = np.zeros(G + 1, np.uint8)
+    sample[0] = 1
+    d = SimDesc(1, 2e7, 42, 7, G, params, mig, None, sample, 15, GF_INIT_PREV, start_generation=100,
+                init_freq=last["freq"], init_ids=a["ids"], init_extinct=np.zeros(1, np.uint8))
+    g, o = gpu.run(d), oracle.run(d)
+    assert_same(g, o, "prev")
+    assert g["samples"][0]["generation"] == 100 and g["samples"][-1]["generation"] == 180
+    assert np.all(g["ids"]["origin_threadID"][: last["num_mutations"]] < 0)  # sampled start -> preserved
+
+
+def test_spectrum_matches_oracle(gpu, oracle):
+    d = S.plain_small(G=50)
+    r = gpu.run(d)
+    row = r["samples"][-1]["freq"][0]
+    Nc = r["samples"][-1]["Nchrom_e"][0]
+    h_g = gpu.population_frequency_histogram(row, Nc, d.num_sites)
+    h_o = oracle.population_frequency_histogram(row, Nc, d.num_sites)
+    assert np.array_equal(h_g, h_o)
+    for n in [2, 50, 100, 1000, 5001]:
+        s_g = gpu.site_frequency_spectrum(row, n, d.num_sites)
+        s_o = oracle.site_frequency_spectrum(row, n, d.num_sites)
+        assert np.array_equal(s_g.view(np.uint32), s_o.view(np.uint32)), n
