@@ -157,14 +157,16 @@ typedef struct gf_batch_result gf_batch_result;
 /* Runs every simulation from mutation-selection equilibrium to its own final
  * generation, one population each, and reports the final SFS of a sample of
  * sfs_n chromosomes per simulation.  Per-simulation results are identical
- * to gf_run on the equivalent gf_sim_desc. */
+ * to gf_run on the equivalent gf_sim_desc.  Generations advance in blocks of
+ * min(compact_interval, 64, gf_set_generations_per_launch) per launch. */
 int32_t gf_batch_run(const gf_batch_sim *sims, int32_t num_sims,
                      int32_t compact_interval, int32_t sfs_n,
-                     int32_t device, gf_batch_result **out);
+                     int32_t keep_states, int32_t device, gf_batch_result **out);
 int32_t gf_batch_sfs(const gf_batch_result *r, float *out /*[num_sims][sfs_n]*/);
 int32_t gf_batch_num_mutations(const gf_batch_result *r, int32_t *out /*[num_sims]*/);
-/* final frequencies and IDs of one simulation, compacted; out arrays sized
- * by gf_batch_num_mutations */
+/* final frequencies and IDs of one simulation, compacted (only when
+ * gf_batch_run was called with keep_states != 0); out arrays sized by
+ * gf_batch_num_mutations */
 int32_t gf_batch_sim_state(const gf_batch_result *r, int32_t sim, float *freq, gf_mutid *ids);
 int32_t gf_batch_live_mutation_generations(const gf_batch_result *r, int64_t *out);
 int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out);

@@ -77,7 +77,8 @@ SIGNATURES = {
     "gf_result_free": (None, [C.c_void_p]),
     "gf_population_histogram": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
-    "gf_batch_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gf_batch_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                 C.POINTER(C.c_void_p)]),
     "gf_batch_sfs": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "gf_batch_num_mutations": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "gf_batch_sim_state": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
@@ -275,4 +276,41 @@ def device_math(op, *inputs, device=-1):
         out = np.empty(n, np.int32 if op in ("binom_draw", "pois_draw") else np.float32)
     ptrs = [None if x is None else x.ctypes.data_as(C.c_void_p) for x in ins]
     check(lib.gf_device_math(code, ptrs[0], ptrs[1], ptrs[2], ptrs[3], out.ctypes.data_as(C.c_void_p), n, device))
+    return out
+
+
+def batch_run(sims, compact_interval=35, sfs_n=50, keep_states=False, device=-1, timing=False):
+    """Runs many single-population simulations in one batch.
+
+    sims: numpy array of BATCH_SIM_DTYPE.  Returns dict with per-simulation
+    SFS [S][sfs_n], final mutation counts, total live mutation-generations,
+    optionally final states (freq, ids) and timing."""
+    lib = load()
+    arr = np.ascontiguousarray(sims, BATCH_SIM_DTYPE)
+    S = len(arr)
+    h = C.c_void_p()
+    check(lib.gf_batch_run(arr.ctypes.data_as(C.c_void_p), S, int(compact_interval), int(sfs_n),
+                           1 if keep_states else 0, device, C.byref(h)))
+    try:
+        sfs = np.empty((S, sfs_n), np.float32)
+        check(lib.gf_batch_sfs(h, sfs.ctypes.data_as(C.c_void_p)))
+        nm = np.empty(S, np.int32)
+        check(lib.gf_batch_num_mutations(h, nm.ctypes.data_as(C.c_void_p)))
+        live = C.c_int64()
+        check(lib.gf_batch_live_mutation_generations(h, C.byref(live)))
+        out = dict(sfs=sfs, num_mutations=nm, live_mutation_generations=live.value)
+        if keep_states:
+            states = []
+            for s in range(S):
+                f = np.empty(nm[s], np.float32)
+                i = np.empty(nm[s], MUTID_DTYPE)
+                check(lib.gf_batch_sim_state(h, s, f.ctypes.data_as(C.c_void_p), i.ctypes.data_as(C.c_void_p)))
+                states.append((f, i))
+            out["states"] = states
+        if timing:
+            t = gf_timing()
+            lib.gf_batch_timing(h, C.byref(t))
+            out["timing"] = {k: getattr(t, k) for k, _ in gf_timing._fields_}
+    finally:
+        lib.gf_batch_free(h)
     return out
