@@ -181,7 +181,8 @@ void    gf_batch_free(gf_batch_result *r);
  *   EXP, LOG, NORMCDFINV, SQRT, ROUND_AWAY: in0 float       out: float
  *   DIV        in0 / in1 (float)                             out: float
  *   BINOM_DRAW in0 uint32 bits, in1 mean, in2 var, in3 N     out: int32
- *   POIS_DRAW  in0 uint32 bits, in1 mean, in2 N              out: int32 */
+ *   POIS_DRAW  in0 uint32 bits, in1 mean, in2 N              out: int32
+ *   COPY       in0 float                                     out: float */
 #define GF_MATH_PHILOX      0
 #define GF_MATH_U01         1
 #define GF_MATH_EXP         2
@@ -192,6 +193,7 @@ void    gf_batch_free(gf_batch_result *r);
 #define GF_MATH_SQRT        7
 #define GF_MATH_DIV         8
 #define GF_MATH_ROUND_AWAY  9
+#define GF_MATH_COPY       10   /* out = in0 (float): a plain copy kernel for counter calibration */
 int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void *in2, const void *in3,
                        void *out, int64_t n, int32_t device);
 

@@ -255,7 +255,7 @@ def set_generations_per_launch(k):
 
 
 MATH_OPS = dict(philox=0, u01=1, exp=2, log=3, normcdfinv=4, binom_draw=5, pois_draw=6, sqrt=7, div=8,
-                round_away=9)
+                round_away=9, copy=10)
 
 
 def device_math(op, *inputs, device=-1):

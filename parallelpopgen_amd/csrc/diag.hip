@@ -34,6 +34,7 @@ static __global__ void k_math(int op, const void *in0, const void *in1, const vo
         case GF_MATH_SQRT: fo[i] = __builtin_sqrtf(f0[i]); break;
         case GF_MATH_DIV: fo[i] = f0[i] / f1[i]; break;
         case GF_MATH_ROUND_AWAY: fo[i] = round_away(f0[i]); break;
+        case GF_MATH_COPY: fo[i] = f0[i]; break;
         default: break;
         }
     }
@@ -44,15 +45,16 @@ static __global__ void k_math(int op, const void *in0, const void *in1, const vo
 extern "C" int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void *in2, const void *in3,
                                   void *out, int64_t n, int32_t device) {
     using namespace gf;
-    if (n <= 0 || !in0 || !out || op < 0 || op > GF_MATH_ROUND_AWAY) { set_error("gf_device_math: bad arguments"); return GF_ERR_ARG; }
+    if (n <= 0 || !in0 || !out || op < 0 || op > GF_MATH_COPY) { set_error("gf_device_math: bad arguments"); return GF_ERR_ARG; }
     GF_TRY(select_device(device));
     const size_t in_bytes = op == GF_MATH_PHILOX ? 24 : 4;
     const size_t out_bytes = op == GF_MATH_PHILOX ? 16 : 4;
     DevBuf<char> d0, d1, d2, d3, dout;
     GF_TRY(d0.alloc(n * in_bytes));
-    GF_TRY(d1.alloc(n * 4));
-    GF_TRY(d2.alloc(n * 4));
-    GF_TRY(d3.alloc(n * 4));
+    const bool extra = op == GF_MATH_DIV || op == GF_MATH_BINOM_DRAW || op == GF_MATH_POIS_DRAW;
+    GF_TRY(d1.alloc(extra ? n * 4 : 1));
+    GF_TRY(d2.alloc(extra ? n * 4 : 1));
+    GF_TRY(d3.alloc(extra ? n * 4 : 1));
     GF_TRY(dout.alloc(n * out_bytes));
     GF_HIP(hipMemcpy(d0.p, in0, n * in_bytes, hipMemcpyHostToDevice));
     if (in1) GF_HIP(hipMemcpy(d1.p, in1, n * 4, hipMemcpyHostToDevice));

@@ -1,0 +1,75 @@
+"""Batched simulator: every simulation of a batch is bit-identical to the
+plain gf_run of the same parameters, and the batch SFS matches the oracle."""
+import numpy as np
+import pytest
+
+from parallelpopgen_amd import scenarios as S
+from parallelpopgen_amd.capi import BATCH_SIM_DTYPE
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(n, rng, G_scale=0.04, L_scale=1.0):
+    sims = np.zeros(n, BATCH_SIM_DTYPE)
+    N = np.exp(rng.uniform(np.log(500), np.log(5000), n)).astype(np.int32)
+    gamma = rng.uniform(-50, 10, n)
+    F = rng.uniform(0, 0.2, n).astype(np.float32)
+    sims["N"] = N
+    sims["F"] = F
+    sims["h"] = rng.choice([0.0, 0.5, 1.0], n)
+    sims["mu"] = 1e-9
+    sims["s"] = [S.selection_from_gamma(g, Ni, Fi) for g, Ni, Fi in zip(gamma, N, F)]
+    sims["s"][::5] = 0.0
+    sims["num_sites"] = (np.exp(rng.uniform(np.log(2e6), np.log(5e7), n)) * L_scale).astype(np.float32)
+    sims["seed1"] = 1000 + np.arange(n)
+    sims["seed2"] = 77
+    sims["num_generations"] = np.maximum(1, (4 * N / 10 * G_scale)).astype(np.int32)
+    sims["num_generations"][3] = 0
+    return sims
+
+
+def plain_desc(sim, compact_interval):
+    return S.constant(N=int(sim["N"]), L=float(sim["num_sites"]), G=int(sim["num_generations"]),
+                      mu=float(sim["mu"]), F=float(sim["F"]), h=float(sim["h"]), s=np.float32(sim["s"]),
+                      seed1=int(sim["seed1"]), seed2=int(sim["seed2"]), compact_interval=compact_interval)
+
+
+@pytest.mark.parametrize("interval", [35, 8])
+def test_batch_bit_identical_to_plain(gpu, interval):
+    rng = np.random.default_rng(21)
+    sims = make_batch(24, rng, G_scale=0.5)
+    b = gpu.batch_run(sims, compact_interval=interval, sfs_n=50, keep_states=True)
+    for k, sim in enumerate(sims):
+        d = plain_desc(sim, 35)
+        r = gpu.run(d)
+        last = r["samples"][-1]
+        f, ids = b["states"][k]
+        assert b["num_mutations"][k] == last["num_mutations"], k
+        assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), k
+        assert np.array_equal(ids, r["ids"]), k
+        sfs = gpu.site_frequency_spectrum(last["freq"][0], 50, d.num_sites)
+        assert np.array_equal(b["sfs"][k].view(np.uint32), sfs.view(np.uint32)), k
+
+
+def test_batch_sfs_matches_oracle(gpu, oracle):
+    rng = np.random.default_rng(22)
+    sims = make_batch(6, rng, G_scale=0.2)
+    b = gpu.batch_run(sims, compact_interval=35, sfs_n=50, keep_states=True)
+    for k, sim in enumerate(sims):
+        d = plain_desc(sim, 35)
+        o = oracle.run(d)
+        last = o["samples"][-1]
+        assert np.array_equal(b["states"][k][0].view(np.uint32), last["freq"][0].view(np.uint32)), k
+        s_o = oracle.site_frequency_spectrum(last["freq"][0], 50, d.num_sites)
+        assert np.array_equal(b["sfs"][k].view(np.uint32), s_o.view(np.uint32)), k
+
+
+def test_batch_live_count_matches_plain(gpu):
+    rng = np.random.default_rng(23)
+    sims = make_batch(10, rng, G_scale=0.3)
+    b = gpu.batch_run(sims, compact_interval=35, sfs_n=20)
+    tot = 0
+    for sim in sims:
+        r = gpu.run(plain_desc(sim, 35))
+        tot += int(r["live"][1:].sum())
+    assert b["live_mutation_generations"] == tot
