@@ -27,7 +27,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
-SOURCES = ["gofish.hip", "spectrum.hip", "diag.hip", "batch.hip", "surrogate.hip"]
+SOURCES = ["gofish.hip", "spectrum.hip", "diag.hip", "batch.hip", "surrogate.hip", "spectrum_api.cpp"]
 
 
 def _compile(src):

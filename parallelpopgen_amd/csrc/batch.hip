@@ -310,7 +310,7 @@ struct BatchRun {
                 h_pres[k] = 0;
                 h_total[k] = 0;
             }
-            for (int a = 0; a < na; a++) h_active[a] = active[a];
+            for (int a = 0; a < na; a++) { h_active[a] = active[a]; h_tot[a] = 0; }
             const ptrdiff_t o_tile_first = ntiles, o_tile_num = o_tile_first + nl, o_M = o_tile_num + nl,
                             o_present = o_M + nl, o_pres = o_present + nl, o_total = o_pres + nl,
                             o_active = o_total + nl, o_tot = o_active + na;
@@ -340,7 +340,8 @@ struct BatchRun {
                 GF_HIP(hipGetLastError());
             }
             /* ---- 2. read back survivors and new-mutation totals ---- */
-            GF_HIP(hipMemcpyAsync(h_total, d_meta.p + o_total, ((size_t)nl + na) * sizeof(int), hipMemcpyDeviceToHost, st));
+            /* [total (nl) | active (na) | tot (na)] */
+            GF_HIP(hipMemcpyAsync(h_total, d_meta.p + o_total, ((size_t)nl + 2 * (size_t)na) * sizeof(int), hipMemcpyDeviceToHost, st));
             GF_HIP(hipStreamSynchronize(st));
             /* ---- 3. new layout: survivors + the block's new mutations ---- */
             std::vector<int64_t> nbase(nl);

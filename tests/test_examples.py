@@ -1,0 +1,93 @@
+"""Example programs written against the C++ API (include/go_fish.cuh,
+spectrum.h, go_fish_data_struct.h): they build with hipcc (and g++ for the
+plain C++ driver), and on the GPU reproduce the reference's published
+example outputs statistically."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(REPO, "examples", "_build")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def test_examples_build():
+    r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "examples")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    for exe in ("speed", "dadi", "compilation"):
+        assert os.access(os.path.join(BUILD, exe), os.X_OK)
+
+
+def run_example(name, *args, timeout=600):
+    exe = os.path.join(BUILD, name)
+    if not os.path.exists(exe):
+        pytest.fail("example %s is not built (make -C examples)" % name)
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout, cwd=BUILD)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+def parse_dadi(out):
+    mean, sd, tail = [], [], {}
+    for line in out.splitlines():
+        t = line.split("\t")
+        if len(t) == 4 and t[0].isdigit():
+            mean.append(float(t[1]))
+            sd.append(float(t[2]))
+        elif ":" in line:
+            k, v = line.split(":", 1)
+            try:
+                tail[k.strip()] = float(v)
+            except ValueError:
+                pass
+    return np.array(mean), np.array(sd), tail
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,key", [(2e9, "run_980"), (2e8, "run_980_2")])
+def test_dadi_scenario_matches_reference_runs(L, key):
+    """50 runs of the dadi scenario: every normalised SFS bin agrees with the
+    reference's 50-run mean within the combined standard error (|z| < 4.5
+    for every bin, and the mean squared z within [0.6, 1.6]); the mean
+    number of mutations in the SFS agrees within 1 %."""
+    ref = json.load(open(os.path.join(GOLDEN, "dadi_sfs.json")))[key]
+    m, s, tail = parse_dadi(run_example("dadi", L, 50))
+    rm, rs = np.array(ref["mean"]), np.array(ref["sd"])
+    assert len(m) == len(rm) == 1000
+    z = (m - rm) / np.sqrt(s ** 2 / 50 + rs ** 2 / 50)
+    assert np.all(np.abs(z) < 4.5), (np.argmax(np.abs(z)) + 1, z[np.argmax(np.abs(z))])
+    assert 0.6 < np.mean(z ** 2) < 1.6, np.mean(z ** 2)
+    nm = tail["average number of mutations in SFS"]
+    assert abs(nm / ref["average_number_of_mutations_in_SFS"] - 1) < 0.01, nm
+
+
+@pytest.mark.gpu
+def test_speed_example_output():
+    ref = json.load(open(os.path.join(GOLDEN, "speed_snps.json")))
+    vals = [r["snps"] for g in ("780M", "980") for r in ref[g] if r["xL"] == 20]
+    out = run_example("speed", 20, 15)
+    n = int([l for l in out.splitlines() if l.startswith("number of mutations")][0].split("\t")[1])
+    assert abs(n - np.mean(vals)) < 4.5 * np.sqrt(np.mean(vals)) * 1.25, (n, vals)
+
+
+@pytest.mark.gpu
+def test_mixed_compiler_example(tmp_path):
+    out = run_example("compilation", str(tmp_path / "bfile.dat"))
+    lines = out.splitlines()
+    ns = int([l for l in lines if l.startswith("number of time samples")][0].split(":")[1])
+    m0 = int([l for l in lines if l.startswith("mutations in first time sample")][0].split(":")[1])
+    m1 = int([l for l in lines if l.startswith("mutations in final time sample")][0].split(":")[1])
+    assert ns == 2
+    # reference example_compilation.out: 5288 and 16736 (a single run)
+    assert 0.7 * 5288 < m0 < 1.3 * 5288 and 0.7 * 16736 < m1 < 1.3 * 16736, (m0, m1)
+    text = open(tmp_path / "bfile.dat").read().splitlines()
+    assert text[0].startswith("seed1\t-1091571699") and text[1].startswith("seed2\t-557122643")
+    assert text[10].startswith("time sample:") and text[11].split("\t")[1:3] == ["20000", ""]
+    rows = [l for l in text if l.startswith("(")]
+    assert len(rows) == m1
+    freqs = np.array([[float(x) for x in r.split("\t")[1:]] for r in rows[:500]])
+    assert freqs.shape[1] == 4
+    assert np.allclose(np.round(freqs * 1000), freqs * 1000, atol=1e-3)   # 1000 haploid chromosomes
