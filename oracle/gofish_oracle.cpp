@@ -216,13 +216,37 @@ static const signed char exit_mean[70] = {
     -1, 27, 28, -1, 29, 30, -1, 31, 32, -1      /* 60-69 */
 };
 
-/* Poisson CDF inversion, reference shared.cuh:168-249 */
+/* correctly rounded reciprocals fl(1/j), j = 1..69 */
+static const float recip[69] = {
+    0x1.0000000000000p+0f, 0x1.0000000000000p-1f, 0x1.5555560000000p-2f, 0x1.0000000000000p-2f,
+    0x1.99999a0000000p-3f, 0x1.5555560000000p-3f, 0x1.24924a0000000p-3f, 0x1.0000000000000p-3f,
+    0x1.c71c720000000p-4f, 0x1.99999a0000000p-4f, 0x1.745d180000000p-4f, 0x1.5555560000000p-4f,
+    0x1.3b13b20000000p-4f, 0x1.24924a0000000p-4f, 0x1.1111120000000p-4f, 0x1.0000000000000p-4f,
+    0x1.e1e1e20000000p-5f, 0x1.c71c720000000p-5f, 0x1.af286c0000000p-5f, 0x1.99999a0000000p-5f,
+    0x1.8618620000000p-5f, 0x1.745d180000000p-5f, 0x1.642c860000000p-5f, 0x1.5555560000000p-5f,
+    0x1.47ae140000000p-5f, 0x1.3b13b20000000p-5f, 0x1.2f684c0000000p-5f, 0x1.24924a0000000p-5f,
+    0x1.1a7b960000000p-5f, 0x1.1111120000000p-5f, 0x1.0842100000000p-5f, 0x1.0000000000000p-5f,
+    0x1.f07c200000000p-6f, 0x1.e1e1e20000000p-6f, 0x1.d41d420000000p-6f, 0x1.c71c720000000p-6f,
+    0x1.bacf920000000p-6f, 0x1.af286c0000000p-6f, 0x1.a41a420000000p-6f, 0x1.99999a0000000p-6f,
+    0x1.8f9c180000000p-6f, 0x1.8618620000000p-6f, 0x1.7d05f40000000p-6f, 0x1.745d180000000p-6f,
+    0x1.6c16c20000000p-6f, 0x1.642c860000000p-6f, 0x1.5c98820000000p-6f, 0x1.5555560000000p-6f,
+    0x1.4e5e0a0000000p-6f, 0x1.47ae140000000p-6f, 0x1.4141420000000p-6f, 0x1.3b13b20000000p-6f,
+    0x1.3521d00000000p-6f, 0x1.2f684c0000000p-6f, 0x1.29e4120000000p-6f, 0x1.24924a0000000p-6f,
+    0x1.1f70480000000p-6f, 0x1.1a7b960000000p-6f, 0x1.15b1e60000000p-6f, 0x1.1111120000000p-6f,
+    0x1.0c97140000000p-6f, 0x1.0842100000000p-6f, 0x1.0410420000000p-6f, 0x1.0000000000000p-6f,
+    0x1.f81f820000000p-7f, 0x1.f07c200000000p-7f, 0x1.e9131a0000000p-7f, 0x1.e1e1e20000000p-7f,
+    0x1.dae6080000000p-7f,
+};
+
+/* Poisson CDF inversion, reference shared.cuh:168-249.  The recurrence
+ * multiplies by fl(1/j) where the reference divides by j (its
+ * --use_fast_math build evaluates that quotient as x * rcp(j) as well). */
 static int poiscdfinv(float r, float mean) {
     float emu = exp_f(-1.0f * mean);
     float cdf = emu;
     if (cdf >= r) return 0;
     for (int j = 1; j < 70; j++) {
-        emu = emu * (mean / (float)j);
+        emu = emu * (mean * recip[j - 1]);
         cdf = cdf + emu;
         if (cdf >= r) return j;
         if (exit_mean[j] >= 0 && mean <= (float)exit_mean[j]) return j;

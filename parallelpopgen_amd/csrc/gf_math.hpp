@@ -124,28 +124,35 @@ GF_HD float pow_f(float a, float b) {
  * 1988, 477-484); u clamped to <= 1 - 2^-24 so the result stays finite */
 GF_HD float normcdfinv_f(float p) {
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
-    float q = p - 0.5f;
-    if (__builtin_fabsf(q) <= 0.425f) {
-        float r = 0.180625f - q * q;
-        float num = ((5.9109374720e+01f * r + 1.5929113202e+02f) * r + 5.0434271938e+01f) * r + 3.3871327179e+00f;
-        float den = ((6.7187563600e+01f * r + 7.8757757664e+01f) * r + 1.7895169469e+01f) * r + 1.0f;
-        return q * num / den;
-    }
-    float r = q < 0.0f ? p : 1.0f - p;
-    r = __builtin_sqrtf(-log_f(r));
-    float z;
-    if (r <= 5.0f) {
-        r = r - 1.6f;
-        float num = ((1.7023821103e-01f * r + 1.3067284816e+00f) * r + 2.7568153900e+00f) * r + 1.4234372777e+00f;
-        float den = (1.2021132975e-01f * r + 7.3700164250e-01f) * r + 1.0f;
-        z = num / den;
+    const float q = p - 0.5f;
+    const bool central = __builtin_fabsf(q) <= 0.425f;
+    /* one rational evaluation with per-lane coefficients; the tail's
+     * sqrt(-log(.)) only runs for lanes outside the central region */
+    float r;
+    float a0, a1, a2, a3, b1, b2, b3;
+    if (central) {
+        r = 0.180625f - q * q;
+        a3 = 5.9109374720e+01f; a2 = 1.5929113202e+02f; a1 = 5.0434271938e+01f; a0 = 3.3871327179e+00f;
+        b3 = 6.7187563600e+01f; b2 = 7.8757757664e+01f; b1 = 1.7895169469e+01f;
     } else {
-        r = r - 5.0f;
-        float num = ((1.7337203997e-02f * r + 4.2868294337e-01f) * r + 3.0812263860e+00f) * r + 6.6579046435e+00f;
-        float den = (1.2258202635e-02f * r + 2.4197894225e-01f) * r + 1.0f;
-        z = num / den;
+        float t = q < 0.0f ? p : 1.0f - p;
+        t = __builtin_sqrtf(-log_f(t));
+        b3 = 0.0f;
+        if (t <= 5.0f) {
+            r = t - 1.6f;
+            a3 = 1.7023821103e-01f; a2 = 1.3067284816e+00f; a1 = 2.7568153900e+00f; a0 = 1.4234372777e+00f;
+            b2 = 1.2021132975e-01f; b1 = 7.3700164250e-01f;
+        } else {
+            r = t - 5.0f;
+            a3 = 1.7337203997e-02f; a2 = 4.2868294337e-01f; a1 = 3.0812263860e+00f; a0 = 6.6579046435e+00f;
+            b2 = 1.2258202635e-02f; b1 = 2.4197894225e-01f;
+        }
     }
-    return q < 0.0f ? -z : z;
+    const float num = ((a3 * r + a2) * r + a1) * r + a0;
+    /* b3 = 0 in the tails: 0*r + b2 == b2 exactly */
+    const float den = ((b3 * r + b2) * r + b1) * r + 1.0f;
+    const float z = ((central ? q : 1.0f) * num) / den;
+    return (central || q >= 0.0f) ? z : -z;
 }
 
 GF_HD float round_away(float v) {
@@ -179,15 +186,39 @@ GF_HD int exit_mean(int j) {
     }
 }
 
+/* correctly rounded reciprocals fl(1/j), j = 1..69, of the Poisson CDF
+ * recurrence emu_j = emu_(j-1) * (mean * (1/j)) */
+static constexpr float kRecip[69] = {
+    0x1.0000000000000p+0f, 0x1.0000000000000p-1f, 0x1.5555560000000p-2f, 0x1.0000000000000p-2f,
+    0x1.99999a0000000p-3f, 0x1.5555560000000p-3f, 0x1.24924a0000000p-3f, 0x1.0000000000000p-3f,
+    0x1.c71c720000000p-4f, 0x1.99999a0000000p-4f, 0x1.745d180000000p-4f, 0x1.5555560000000p-4f,
+    0x1.3b13b20000000p-4f, 0x1.24924a0000000p-4f, 0x1.1111120000000p-4f, 0x1.0000000000000p-4f,
+    0x1.e1e1e20000000p-5f, 0x1.c71c720000000p-5f, 0x1.af286c0000000p-5f, 0x1.99999a0000000p-5f,
+    0x1.8618620000000p-5f, 0x1.745d180000000p-5f, 0x1.642c860000000p-5f, 0x1.5555560000000p-5f,
+    0x1.47ae140000000p-5f, 0x1.3b13b20000000p-5f, 0x1.2f684c0000000p-5f, 0x1.24924a0000000p-5f,
+    0x1.1a7b960000000p-5f, 0x1.1111120000000p-5f, 0x1.0842100000000p-5f, 0x1.0000000000000p-5f,
+    0x1.f07c200000000p-6f, 0x1.e1e1e20000000p-6f, 0x1.d41d420000000p-6f, 0x1.c71c720000000p-6f,
+    0x1.bacf920000000p-6f, 0x1.af286c0000000p-6f, 0x1.a41a420000000p-6f, 0x1.99999a0000000p-6f,
+    0x1.8f9c180000000p-6f, 0x1.8618620000000p-6f, 0x1.7d05f40000000p-6f, 0x1.745d180000000p-6f,
+    0x1.6c16c20000000p-6f, 0x1.642c860000000p-6f, 0x1.5c98820000000p-6f, 0x1.5555560000000p-6f,
+    0x1.4e5e0a0000000p-6f, 0x1.47ae140000000p-6f, 0x1.4141420000000p-6f, 0x1.3b13b20000000p-6f,
+    0x1.3521d00000000p-6f, 0x1.2f684c0000000p-6f, 0x1.29e4120000000p-6f, 0x1.24924a0000000p-6f,
+    0x1.1f70480000000p-6f, 0x1.1a7b960000000p-6f, 0x1.15b1e60000000p-6f, 0x1.1111120000000p-6f,
+    0x1.0c97140000000p-6f, 0x1.0842100000000p-6f, 0x1.0410420000000p-6f, 0x1.0000000000000p-6f,
+    0x1.f81f820000000p-7f, 0x1.f07c200000000p-7f, 0x1.e9131a0000000p-7f, 0x1.e1e1e20000000p-7f,
+    0x1.dae6080000000p-7f,
+};
+
 GF_HD int poiscdfinv(float r, float mean) {
     float emu = exp_f(-1.0f * mean);
     float cdf = emu;
     if (cdf >= r) return 0;
+#pragma unroll
     for (int j = 1; j < 70; j++) {
-        emu = emu * (mean / (float)j);
+        emu = emu * (mean * kRecip[j - 1]);
         cdf = cdf + emu;
         if (cdf >= r) return j;
-        int em = exit_mean(j);
+        const int em = exit_mean(j);
         if (em >= 0 && mean <= (float)em) return j;
     }
     return 70;
@@ -209,27 +240,30 @@ GF_HD int binomcdfinv(float r, float mean, float x, float n) {
     return 70;
 }
 
-/* approximate binomial draw (reference shared.cu:10-18) */
+/* approximate binomial draw (reference shared.cu:10-18): the two Poisson
+ * (or binomial, N < 100) tails share one CDF inversion; the rest is the
+ * rounded normal approximation */
 GF_HD int binom_draw(uint32_t bits, float mean, float var, float N) {
-    float u = u01(bits);
-    if (mean <= 6.0f) {
-        if (N < 100.0f) return binomcdfinv(u, mean, mean / N, N);
-        return poiscdfinv(u, mean);
-    } else if (mean >= N - 6.0f) {
-        if (N < 100.0f) {
-            float nm = N - mean;
-            return f2i_sat(N - (float)binomcdfinv(u, nm, nm / N, N));
-        }
-        return f2i_sat(N - (float)poiscdfinv(u, N - mean));
+    const float u = u01(bits);
+    const bool lo = mean <= 6.0f;
+    const bool hi = !lo && (mean >= N - 6.0f);
+    if (lo || hi) {
+        const float m = lo ? mean : N - mean;
+        const int k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv(u, m);
+        return lo ? k : f2i_sat(N - (float)k);
     }
     return f2i_sat(round_away(normcdfinv_f(u) * __builtin_sqrtf(var) + mean));
 }
 
 /* approximate Poisson draw (reference shared.cuh:251-256) */
 GF_HD int pois_draw(uint32_t bits, float mean, float var, float N) {
-    float u = u01(bits);
-    if (mean <= 6.0f) return poiscdfinv(u, mean);
-    if (mean >= N - 6.0f) return f2i_sat(N - (float)poiscdfinv(u, N - mean));
+    const float u = u01(bits);
+    const bool lo = mean <= 6.0f;
+    const bool hi = !lo && (mean >= N - 6.0f);
+    if (lo || hi) {
+        const int k = poiscdfinv(u, lo ? mean : N - mean);
+        return lo ? k : f2i_sat(N - (float)k);
+    }
     return f2i_sat(round_away(normcdfinv_f(u) * __builtin_sqrtf(var) + mean));
 }
 
