@@ -214,6 +214,175 @@ __global__ __launch_bounds__(WG) void k_advance(float *__restrict__ freq, int64_
     }
 }
 
+/* ---------------- fused advance + stable compaction ----------------
+ * One launch advances every slot of a segment by K generations and, when
+ * `compact` is set, writes only the survivors (compaction keep rule at the
+ * last generation of the block) in their original order into the output
+ * buffers; otherwise every slot is written to the same position.  The order
+ * is kept by a single-pass decoupled look-back scan: workgroups take tiles
+ * in increasing order from a ticket counter, publish their tile's survivor
+ * count, and read their predecessors' published (aggregate or inclusive)
+ * counts.  Each published word is (epoch << 34) | (flag << 32) | count with
+ * flag 1 = aggregate, 2 = inclusive prefix; the epoch identifies the launch,
+ * so the status array never needs clearing.  The slot count before the
+ * block is read from the device (M_in) and the count after it is written
+ * there (M_out): the host never needs it to launch the next block. */
+struct FusedLaunch {
+    const float *freq_in;
+    const gf_mutid *ids_in;
+    float *freq_out;
+    gf_mutid *ids_out;
+    int64_t pitch;
+    const int *M_in;
+    int *M_out;
+    unsigned long long *status;   /* one word per tile */
+    unsigned int *ticket;         /* zero before the launch */
+    int *error;                   /* set if a look-back wait exceeded its bound */
+    unsigned long long epoch;
+    int ntiles;                   /* tiles launched (upper bound of the needed ones) */
+    int new_in_block;
+    int compact;
+    int preserve;
+};
+
+constexpr unsigned long long FLAG_AGG = 1ull, FLAG_INC = 2ull;
+
+__device__ __forceinline__ unsigned long long status_word(unsigned long long epoch, unsigned long long flag, unsigned int v) {
+    return (epoch << 34) | (flag << 32) | (unsigned long long)v;
+}
+
+template <int P>
+__global__ __launch_bounds__(WG) void k_advance_fused(const FusedLaunch F, const SegLaunch<P> S) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    __shared__ int sh_tile;
+    __shared__ unsigned int sh_wcount[WG / 64];
+    __shared__ unsigned int sh_excl;
+    __shared__ unsigned long long sh_live[64];
+    if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
+    const int M0 = *F.M_in;
+    const int Mend = M0 + F.new_in_block;
+    const int need_tiles = (Mend + WG - 1) / WG;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (Mend == 0) *F.M_out = 0;
+        if (need_tiles > F.ntiles) *F.error = 2;   /* host bound too small: slots would be dropped */
+    }
+    uint64_t cnt = 0;
+    const GenRow<P> *rows = S.rows;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_tile = (int)atomicAdd(F.ticket, 1u);
+        __syncthreads();
+        const int tile = sh_tile;
+        if (tile >= F.ntiles || tile >= need_tiles) break;
+        const int li = tile * WG + (int)threadIdx.x;
+        const bool valid = li < Mend;
+        float x[P];
+        int ogen = 0, opop = 0, born = -1;
+        uint32_t tid = 0;
+        bool preserved = false;
+#pragma unroll
+        for (int p = 0; p < P; p++) x[p] = 0.0f;
+        gf_mutid id;
+        id.origin_generation = 0; id.origin_population = 0; id.origin_threadID = 0; id.reserved = 0;
+        if (valid) {
+            if (li < M0) {
+#pragma unroll
+                for (int p = 0; p < P; p++) x[p] = F.freq_in[p * F.pitch + li];
+                id = F.ids_in[li];
+                ogen = id.origin_generation;
+                opop = id.origin_population;
+                preserved = id.origin_threadID < 0;
+                tid = (uint32_t)(preserved ? -id.origin_threadID : id.origin_threadID);
+            } else {
+                const int rel = li - M0 + S.cum_base;
+                int tb, pb;
+                find_birth<P>(rows, S.K, rel, tb, pb);
+                born = tb;
+                ogen = S.g0 + 1 + tb;
+                opop = pb;
+                tid = (uint32_t)(rel - rows[tb].newcum[pb] + 1);
+#pragma unroll
+                for (int p = 0; p < P; p++) x[p] = (p == pb) ? rows[tb].newfreq[p] : 0.0f;
+                id.origin_generation = ogen;
+                id.origin_population = opop;
+                id.origin_threadID = (int32_t)tid;
+                id.reserved = 0;
+            }
+        }
+        u32x4 rc[P];
+        bool keep = false;
+        for (int t = 0; t < S.K; t++) {
+            const GenRow<P> &R = rows[t];
+            const int g = S.g0 + 1 + t;
+            if (valid && born < t) {
+                const bool refresh = ((g & 3) == 0) || (t == born + 1);
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop);
+            }
+            keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
+            const uint64_t b = __ballot(keep);
+            if (lane == t) cnt += (uint64_t)__popcll(b);
+        }
+        if (!F.compact) {
+            if (valid) {
+#pragma unroll
+                for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + li] = x[p];
+                F.ids_out[li] = id;
+            }
+            continue;
+        }
+        /* stable compaction of this tile */
+        const uint64_t kb = __ballot(keep);
+        if (lane == 0) sh_wcount[w] = (unsigned int)__popcll(kb);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned int c = 0;
+            for (int k = 0; k < WG / 64; k++) c += sh_wcount[k];
+            unsigned int excl = 0;
+            if (tile == 0) {
+                __hip_atomic_store(&F.status[0], status_word(F.epoch, FLAG_INC, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&F.status[tile], status_word(F.epoch, FLAG_AGG, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int p = tile - 1;
+                long spins = 0;
+                while (p >= 0) {
+                    const unsigned long long v = __hip_atomic_load(&F.status[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long flag = (v >> 32) & 3ull;
+                    if ((v >> 34) != F.epoch || flag == 0ull) {
+                        if (++spins > (1l << 26)) { *F.error = 1; break; }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    excl += (unsigned int)v;
+                    if (flag == FLAG_INC) break;
+                    p--;
+                }
+                __hip_atomic_store(&F.status[tile], status_word(F.epoch, FLAG_INC, excl + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            sh_excl = excl;
+            if (tile == need_tiles - 1) *F.M_out = (int)(excl + c);
+        }
+        __syncthreads();
+        if (keep) {
+            unsigned int pos = sh_excl;
+            for (int k = 0; k < w; k++) pos += sh_wcount[k];
+            pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(kb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)kb, 0u));
+#pragma unroll
+            for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + pos] = x[p];
+            if (F.preserve) {
+                const int z = id.origin_threadID;
+                id.origin_threadID = -(z < 0 ? -z : z);
+            }
+            F.ids_out[pos] = id;
+        }
+    }
+    /* per-generation live counts: lane t of every wave holds generation t */
+    if (cnt) atomicAdd(&sh_live[lane], (unsigned long long)cnt);
+    __syncthreads();
+    if ((int)threadIdx.x < S.K && sh_live[threadIdx.x])
+        atomicAdd(&S.live[(size_t)(S.t0 + threadIdx.x) * 64 + (blockIdx.x & 63)], sh_live[threadIdx.x]);
+}
+
 /* ---------------- compaction (stable stream compaction) ---------------- */
 
 /* flag pass: one bit per slot (wave64 ballot), one count per 256-slot tile

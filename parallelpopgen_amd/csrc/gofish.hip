@@ -149,6 +149,17 @@ struct PlainRun {
     size_t tile_seg_n = 0;
     KernelTimer t_adv, t_cmp;
     int64_t slot_gens = 0;
+    /* fused advance + compaction state */
+    DevBuf<int> d_M;                    /* [2] slot counts, alternately in and out */
+    int mcur = 0;                       /* d_M[mcur] = current slot count */
+    DevBuf<unsigned long long> d_status;
+    DevBuf<unsigned int> d_tickets;     /* one ticket counter per launch */
+    DevBuf<int> d_err;
+    PinBuf<int> h_ring;                 /* slot count after each launch */
+    std::vector<hipEvent_t> ev_ring;
+    int64_t launches = 0;
+    int64_t M_bound = 0;                /* upper bound of the current slot count */
+    int64_t last_new = 0;               /* new slots of the most recent launch */
 
     ~PlainRun() { if (st) (void)hipStreamDestroy(st); }
 
@@ -413,35 +424,90 @@ struct PlainRun {
         return GF_OK;
     }
 
-    int launch_block(int t_first, int K) {
-        /* rows t_first .. t_first+K-1 ; generation before the block = start + t_first - 1 */
+    static constexpr int RING = 4;
+
+    /* exact slot count: wait for the stream */
+    int exact_M(int &out) {
+        GF_HIP(hipMemcpyAsync(&h_ring.p[RING], d_M.p + mcur, sizeof(int), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        out = h_ring.p[RING];
+        M_bound = out;
+        return GF_OK;
+    }
+
+    /* grow the buffers to hold `need` slots; the current contents (exact
+     * count read from the device) move along */
+    int grow(int64_t need) {
+        if (need <= cap && freqA.p) return GF_OK;
+        int cur = 0;
+        GF_TRY(exact_M(cur));
+        M = cur;
+        return ensure_capacity(need);
+    }
+
+    /* Advance generations t_first .. t_first+K-1 in one fused launch
+     * (reference migration_selection_drift + add_new_mutations per
+     * generation, go_fish_impl.cuh:645-684, and, when `compact`,
+     * compact :370-429 at the block's last generation). */
+    int launch_block(int t_first, int K, bool compact, bool preserve) {
         const int64_t new_in_block = (int64_t)rows[t_first + K - 1].newcum[P] - rows[t_first].newcum[0];
-        const int64_t Mend = (int64_t)M + new_in_block;
-        if (Mend > 2000000000LL) { set_error("more than 2e9 mutation slots are not supported"); return GF_ERR_UNSUPPORTED; }
-        GF_TRY(ensure_capacity(Mend));
-        if (Mend > 0) {
-            SegLaunch<P> sl{};
-            sl.rows = d_rows.p + t_first;
-            sl.live = d_live.p;
-            sl.base = 0;
-            sl.M0 = M;
-            sl.Mend = (int)Mend;
-            sl.cum_base = rows[t_first].newcum[0];
-            sl.K = K;
-            sl.g0 = start + t_first - 1;
-            sl.t0 = t_first;
-            sl.k0 = k0; sl.k1 = k1;
-            const int tiles = (int)((Mend + WG - 1) / WG);
-            const int grid = std::min(tiles, num_cus() * 8);
-            hipEvent_t ev = nullptr;
-            t_adv.begin(st, ev);
-            hipLaunchKernelGGL((k_advance<P, false>), dim3(grid), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
-                               sl, (const SegLaunch<P> *)nullptr, (const int *)nullptr, (const int *)nullptr, tiles);
-            GF_HIP(hipGetLastError());
-            t_adv.end(ev);
-            slot_gens += (int64_t)M * K + new_in_block * K / 2;
+        int64_t bound = M_bound + new_in_block;
+        if (bound > 2000000000LL) { set_error("more than 2e9 mutation slots are not supported"); return GF_ERR_UNSUPPORTED; }
+        if (bound > cap) {
+            GF_TRY(grow(bound + bound / 2));
+            bound = M_bound + new_in_block;
         }
-        M = (int)Mend;
+        const int ntiles = (int)((bound + WG - 1) / WG);
+        if ((size_t)ntiles > d_status.n) {
+            GF_TRY(d_status.alloc((size_t)ntiles + (size_t)ntiles / 2 + 1024));
+            GF_HIP(hipMemsetAsync(d_status.p, 0, d_status.n * sizeof(unsigned long long), st));
+        }
+        FusedLaunch F{};
+        F.freq_in = freqA.p; F.ids_in = idsA.p; F.freq_out = freqB.p; F.ids_out = idsB.p;
+        F.pitch = cap;
+        F.M_in = d_M.p + mcur;
+        F.M_out = d_M.p + (mcur ^ 1);
+        F.status = d_status.p;
+        F.ticket = d_tickets.p + launches;
+        F.error = d_err.p;
+        F.epoch = (unsigned long long)(launches + 1);
+        F.ntiles = ntiles;
+        F.new_in_block = (int)new_in_block;
+        F.compact = compact ? 1 : 0;
+        F.preserve = preserve ? 1 : 0;
+        SegLaunch<P> sl{};
+        sl.rows = d_rows.p + t_first;
+        sl.live = d_live.p;
+        sl.base = 0;
+        sl.cum_base = rows[t_first].newcum[0];
+        sl.K = K;
+        sl.g0 = start + t_first - 1;
+        sl.t0 = t_first;
+        sl.k0 = k0; sl.k1 = k1;
+        const int grid = std::max(1, std::min(ntiles, num_cus() * 8));
+        hipEvent_t ev = nullptr;
+        t_adv.begin(st, ev);
+        hipLaunchKernelGGL((k_advance_fused<P>), dim3(grid), dim3(WG), 0, st, F, sl);
+        GF_HIP(hipGetLastError());
+        t_adv.end(ev);
+        freqA.swap(freqB);
+        idsA.swap(idsB);
+        mcur ^= 1;
+        /* pipelined exact count: copy this launch's slot count to the host
+         * and tighten the bound with the previous launch's count */
+        const int slot = (int)(launches % RING);
+        GF_HIP(hipMemcpyAsync(&h_ring.p[slot], d_M.p + mcur, sizeof(int), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipEventRecord(ev_ring[slot], st));
+        if (launches > 0) {
+            const int prev = (int)((launches - 1) % RING);
+            GF_HIP(hipEventSynchronize(ev_ring[prev]));
+            M_bound = (int64_t)h_ring.p[prev] + new_in_block;
+        } else {
+            M_bound = bound;
+        }
+        slot_gens += M_bound * K;
+        last_new = new_in_block;
+        launches++;
         return GF_OK;
     }
 
@@ -484,12 +550,29 @@ struct PlainRun {
         }
         if (d->sample[0] && start != final_g) GF_TRY(take_sample(0, false));
 
+        /* fused-launch state */
+        GF_TRY(d_M.alloc(2));
+        GF_HIP(hipMemcpyAsync(d_M.p, &M, sizeof(int), hipMemcpyHostToDevice, st));
+        GF_TRY(d_tickets.alloc((size_t)G + 2));
+        GF_HIP(hipMemsetAsync(d_tickets.p, 0, ((size_t)G + 2) * sizeof(unsigned int), st));
+        GF_TRY(d_err.alloc(1));
+        GF_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), st));
+        GF_TRY(h_ring.alloc(RING + 1));
+        ev_ring.assign(RING, nullptr);
+        for (auto &e : ev_ring) GF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        GF_HIP(hipStreamSynchronize(st));
+        mcur = 0;
+        M_bound = M;
+
         int next_compact = start + interval;
         int t = 0;   /* generations done */
         const int kpl = std::max(1, std::min(KMAX, generations_per_launch()));
+        /* one population: a lost or fixed mutation never changes again, so
+         * compacting every launch leaves every reported sample unchanged */
+        const bool compact_every_launch = (P == 1 && interval > 0);
+        int rc = GF_OK;
         while (t < G) {
             int K = 0;
-            int tb = t + 1;
             for (;;) {
                 K++;
                 const int te = t + K;
@@ -498,15 +581,30 @@ struct PlainRun {
                 const bool cmp = interval > 0 && (g == next_compact || g == final_g || pres);
                 if (cmp || d->sample[te] || te == G || K == kpl) break;
             }
-            GF_TRY(launch_block(tb, K));
-            t += K;
-            const int g = start + t;
-            const bool pres = d->preserve[t] || d->sample[t];
-            if ((g == next_compact || g == final_g || pres) && interval > 0) {
-                GF_TRY(compact(rows[t].present_mask, pres));
-                next_compact = g + interval;
+            const int te = t + K;
+            const int g = start + te;
+            const bool pres = d->preserve[te] || d->sample[te];
+            const bool user_cmp = interval > 0 && (g == next_compact || g == final_g || pres);
+            rc = launch_block(t + 1, K, user_cmp || compact_every_launch, user_cmp && pres);
+            if (rc) break;
+            t = te;
+            if (user_cmp) next_compact = g + interval;
+            if (d->sample[t] && g != final_g) {
+                int cur = 0;
+                if ((rc = exact_M(cur))) break;
+                M = cur;
+                if ((rc = take_sample(t, false))) break;
             }
-            if (d->sample[t] && g != final_g) GF_TRY(take_sample(t, false));
+        }
+        for (auto &e : ev_ring) if (e) (void)hipEventDestroy(e);
+        if (rc) return rc;
+        {
+            int cur = 0;
+            GF_TRY(exact_M(cur));
+            M = cur;
+            int err = 0;
+            GF_HIP(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
+            if (err) { set_error("advance kernel: look-back wait exceeded its bound"); return GF_ERR_DEVICE; }
         }
         GF_TRY(take_sample(G, true));
         GF_HIP(hipStreamSynchronize(st));
