@@ -264,7 +264,8 @@ __global__ __launch_bounds__(WG) void k_advance_fused(const FusedLaunch F, const
     const int Mend = M0 + F.new_in_block;
     const int need_tiles = (Mend + WG - 1) / WG;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (Mend == 0) *F.M_out = 0;
+        /* without compaction every slot is kept in place */
+        if (Mend == 0 || !F.compact) *F.M_out = Mend;
         if (need_tiles > F.ntiles) *F.error = 2;   /* host bound too small: slots would be dropped */
     }
     uint64_t cnt = 0;

@@ -209,22 +209,42 @@ static constexpr float kRecip[69] = {
     0x1.dae6080000000p-7f,
 };
 
+/* First term index at which the truncated inversion stops whatever the
+ * CDF: the smallest j with exit_mean(j) >= mean, 70 if none.  exit_mean is
+ * increasing in j, so for mean <= 32 this is the j with
+ * exit_mean(j) = max(1, ceil(mean)). */
+static constexpr signed char kExitIndex[33] = {
+    11, 11, 14, 17, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 41, 43,
+    45, 46, 48, 50, 51, 53, 55, 56, 58, 59, 61, 62, 64, 65, 67, 68};
+
+GF_HD int exit_index(float mean) {
+    if (!(mean <= 32.0f)) return 70;
+    int k = (int)__builtin_ceilf(mean);
+    k = k < 1 ? 1 : k;
+    if (k <= 4) return 8 + 3 * k;
+    if (k <= 14) return 12 + 2 * k;
+    return kExitIndex[k];
+}
+
 GF_HD int poiscdfinv(float r, float mean) {
     float emu = exp_f(-1.0f * mean);
     float cdf = emu;
     if (cdf >= r) return 0;
+    const int jstop = exit_index(mean);
+    int j = 1;
 #pragma unroll
-    for (int j = 1; j < 70; j++) {
-        emu = emu * (mean * kRecip[j - 1]);
+    for (int jj = 1; jj < 70; jj++) {
+        emu = emu * (mean * kRecip[jj - 1]);
         cdf = cdf + emu;
-        if (cdf >= r) return j;
-        const int em = exit_mean(j);
-        if (em >= 0 && mean <= (float)em) return j;
+        j = jj;
+        if (cdf >= r || jj == jstop) return j;
     }
     return 70;
 }
 
-GF_HD int binomcdfinv(float r, float mean, float x, float n) {
+/* binomial CDF inversion (N < 100 only): kept out of line, it is off the
+ * hot path */
+static __host__ __device__ __attribute__((noinline)) int binomcdfinv(float r, float mean, float x, float n) {
     float emu = pow_f(1.0f - x, n);
     if (emu == 1.0f) emu = exp_f(-1.0f * mean);
     float cdf = emu;
