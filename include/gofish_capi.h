@@ -144,7 +144,8 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t num_mutations,
  * back-to-back run_sim + site_frequency_spectrum loop of
  * examples/example_dadi/run.cu:139-150) ----- */
 typedef struct gf_batch_sim {
-    int32_t N;                 /* individuals, constant in time */
+    int32_t N;                 /* individuals at generation 0 (equilibrium start) */
+    int32_t N1;                /* individuals from generation 1 on (<= 0: N); one instantaneous size change */
     float   F, h, mu;
     float   s;                 /* constant selection coefficient */
     float   num_sites;
@@ -171,6 +172,36 @@ int32_t gf_batch_sim_state(const gf_batch_result *r, int32_t sim, float *freq, g
 int32_t gf_batch_live_mutation_generations(const gf_batch_result *r, int64_t *out);
 int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out);
 void    gf_batch_free(gf_batch_result *r);
+
+/* ----- learned surrogate of the sample SFS (no reference counterpart) -----
+ * An ensemble of `ensemble` MLPs, 6 inputs -> 4 hidden layers of 512 (GELU)
+ * -> 99 softmax bins (normalised SFS of an n = 100 sample, bins 1..99) + a
+ * log-total head; BF16 weights, FP32 accumulation, trained by the library's
+ * own MFMA forward / backward and Adam kernels.  Inputs per example:
+ * log(N1/N0), growth time, log|gamma|, sign(gamma), h, F. */
+typedef struct gf_sg_model gf_sg_model;
+int32_t gf_sg_create(int32_t ensemble, uint64_t seed, int32_t device, gf_sg_model **out);
+void    gf_sg_free(gf_sg_model *model);
+/* parameters per ensemble member (floats); get/set copy [ensemble][n] */
+int32_t gf_sg_num_params(const gf_sg_model *model);
+int32_t gf_sg_get_params(gf_sg_model *model, float *out);
+int32_t gf_sg_set_params(gf_sg_model *model, const float *in);
+/* one forward + backward pass (no update), batch a multiple of 64:
+ * gradients [ensemble][n_params] and per-member loss */
+int32_t gf_sg_gradients(gf_sg_model *model, const float *x /*[B][6]*/, const float *y /*[B][99]*/,
+                        const float *log_total /*[B]*/, int32_t batch, float total_weight,
+                        float *grad_out, float *loss_out);
+/* `epochs` passes of Adam over n examples in mini-batches (a multiple of
+ * 64) visited in `order` ([epochs][n] row indices, or NULL); mean loss per
+ * epoch */
+int32_t gf_sg_train(gf_sg_model *model, const float *x, const float *y, const float *log_total, int32_t n,
+                    int32_t epochs, int32_t batch, float lr, float total_weight, const int32_t *order,
+                    float *epoch_loss);
+/* ensemble mean SFS [n][99], mean log-total [n], disagreement [n] */
+int32_t gf_sg_predict(gf_sg_model *model, const float *x, int32_t n, float *sfs, float *log_total,
+                      float *disagreement);
+/* raw logits of every member, n a multiple of 64: [ensemble][n][128] */
+int32_t gf_sg_forward(gf_sg_model *model, const float *x, int32_t n, float *logits);
 
 /* ----- diagnostics (no reference counterpart) -----
  * Evaluates the device implementation of one random-number / elementary

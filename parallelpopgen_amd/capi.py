@@ -54,12 +54,12 @@ class gf_timing(C.Structure):
 
 
 class gf_batch_sim(C.Structure):
-    _fields_ = [("N", C.c_int32), ("F", C.c_float), ("h", C.c_float), ("mu", C.c_float),
+    _fields_ = [("N", C.c_int32), ("N1", C.c_int32), ("F", C.c_float), ("h", C.c_float), ("mu", C.c_float),
                 ("s", C.c_float), ("num_sites", C.c_float), ("seed1", C.c_int32), ("seed2", C.c_int32),
                 ("num_generations", C.c_int32)]
 
 
-BATCH_SIM_DTYPE = np.dtype([("N", np.int32), ("F", np.float32), ("h", np.float32), ("mu", np.float32),
+BATCH_SIM_DTYPE = np.dtype([("N", np.int32), ("N1", np.int32), ("F", np.float32), ("h", np.float32), ("mu", np.float32),
                             ("s", np.float32), ("num_sites", np.float32), ("seed1", np.int32),
                             ("seed2", np.int32), ("num_generations", np.int32)])
 
@@ -85,6 +85,17 @@ SIGNATURES = {
     "gf_batch_live_mutation_generations": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     "gf_batch_timing": (C.c_int32, [C.c_void_p, C.POINTER(gf_timing)]),
     "gf_batch_free": (None, [C.c_void_p]),
+    "gf_sg_create": (C.c_int32, [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gf_sg_free": (None, [C.c_void_p]),
+    "gf_sg_num_params": (C.c_int32, [C.c_void_p]),
+    "gf_sg_get_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "gf_sg_set_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "gf_sg_gradients": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float,
+                                    C.c_void_p, C.c_void_p]),
+    "gf_sg_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                C.c_float, C.c_float, C.c_void_p, C.c_void_p]),
+    "gf_sg_predict": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gf_sg_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gf_device_math": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.c_int32]),
     "gf_last_error": (C.c_char_p, []),

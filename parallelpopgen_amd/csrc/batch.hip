@@ -66,6 +66,7 @@ static __global__ void k_batch_rows(const BatchSimDev *__restrict__ sims, const 
         GenRow<1> r;
         r.Nc[0] = S.Nc;
         r.fN[0] = S.fN;
+        r.invN[0] = S.Nc > 0 ? 1.0 / (double)S.fN : 0.0;
         r.sa[0] = 0.0f;
         r.sb[0] = S.s;
         r.c1[0] = S.c1;
@@ -131,16 +132,17 @@ struct BatchRun {
             const gf_batch_sim &b = in[s];
             BatchSimDev &d = sims[s];
             memset(&d, 0, sizeof d);
-            d.N = b.N; d.F = b.F; d.h = b.h; d.mu = b.mu; d.s = b.s; d.L = b.num_sites;
+            const int N1 = b.N1 > 0 ? b.N1 : b.N;
+            d.N = N1; d.F = b.F; d.h = b.h; d.mu = b.mu; d.s = b.s; d.L = b.num_sites;
             d.G = b.num_generations;
             d.k0 = (uint32_t)b.seed1; d.k1 = (uint32_t)b.seed2;
-            d.Nc = b.N > 0 ? nchrom(b.N, b.F) : 0;
+            d.Nc = N1 > 0 ? nchrom(N1, b.F) : 0;
             d.fN = (float)d.Nc;
             d.c1 = (b.F + b.h) - (b.h * b.F);
             d.c2 = (b.F + (2.0f * b.h)) - ((2.0f * b.h) * b.F);
             d.newfreq = d.Nc > 0 ? 1.0f / (float)d.Nc : 0.0f;
             d.neutral = b.s == 0.0f;
-            if (b.N < 0 || b.F < 0 || b.F > 1 || b.mu < 0 || b.num_generations < 0) {
+            if (b.N < 0 || N1 < 0 || b.F < 0 || b.F > 1 || b.mu < 0 || b.num_generations < 0) {
                 set_error("gf_batch_run: simulation " + std::to_string(s) + " has invalid parameters");
                 return GF_ERR_PARAM;
             }
@@ -156,9 +158,10 @@ struct BatchRun {
         int64_t coff = 0, aoff = 0, choff = 0;
         bool any_sel = false;
         for (int s = 0; s < S; s++) {
-            if (sims[s].Nc <= 1) continue;
+            const int Nc0 = in[s].N > 0 ? nchrom(in[s].N, in[s].F) : 0;
+            if (Nc0 <= 1) continue;
             MseSeg m{};
-            m.Nc = sims[s].Nc; m.Nind = in[s].N; m.F = in[s].F; m.h = in[s].h; m.mu = in[s].mu;
+            m.Nc = Nc0; m.Nind = in[s].N; m.F = in[s].F; m.h = in[s].h; m.mu = in[s].mu;
             m.sa = 0.0f; m.sb = in[s].s; m.L = in[s].num_sites;
             m.k0 = sims[s].k0; m.k1 = sims[s].k1; m.pop = 0;
             m.neutral = sims[s].neutral; m.preserve = 0;
@@ -306,7 +309,7 @@ struct BatchRun {
                 h_tile_first[k] = tile_first[k];
                 h_tile_num[k] = tile_num[k];
                 h_M[k] = Mend[s];
-                h_present[k] = in[s].N > 0 ? 1 : 0;
+                h_present[k] = sims[s].N > 0 ? 1 : 0;
                 h_pres[k] = 0;
                 h_total[k] = 0;
             }

@@ -30,6 +30,10 @@ constexpr int WG = 256;    /* threads per workgroup = slots per tile */
  * loads: every field is uniform across a tile). */
 template <int P>
 struct alignas(16) GenRow {
+    double invN[P];       /* 1/(double)fN.  For floats a and b, (float)((double)a * (1/(double)b))
+                           * equals the correctly rounded a/b: a/b has a denominator below 2^24, so
+                           * it is never a float rounding midpoint and lies at least 2^-49 (relative)
+                           * from one, while the double product is within 2^-52 of a/b. */
     int   Nc[P];          /* effective chromosomes; <= 0: population not drifted -> 0 */
     float fN[P];          /* (float)Nc */
     float sa[P], sb[P];   /* selection s(x) = sa*x + sb */
@@ -71,16 +75,26 @@ __device__ __forceinline__ bool keep_rule(const float *x, int present_mask, bool
     return !(zero || one) || preserved;
 }
 
-/* migration + selection + drift of one mutation in one generation for all
- * populations (reference migration_selection_drift, go_fish_impl.cuh:93-124) */
+/* ---------------- migration, selection and drift of one generation ----------------
+ * Reference migration_selection_drift (go_fish_impl.cuh:93-124): each
+ * population's frequency after migration (weighted sum over the source
+ * populations), selection (Eq. 13 of the reference's model, with F and h)
+ * and an approximate binomial(N, x') drift draw, clamped to [0, N].
+ *
+ * rc[p] caches the lane's Philox block for generations 4b..4b+3 (the
+ * reference's float4 packing, go_fish_impl.cuh:107).  It is fetched when
+ * `refresh` is set (at a block boundary, or in the first generation a slot
+ * is drifted in this launch), also for a population that is empty in this
+ * generation, so that it is current if the population is founded later in
+ * the same block. */
 template <int P>
-__device__ __forceinline__ void generation_update(float *x, const GenRow<P> &R, int g, bool refresh,
-                                                  u32x4 *rc, uint32_t k0, uint32_t k1,
-                                                  uint32_t tid, int ogen, int opop) {
+__device__ __forceinline__ void generation_update(float *x, const GenRow<P> &R, int g, bool refresh, u32x4 *rc,
+                                                  uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop) {
     float nx[P];
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int Nc = R.Nc[p];
+        if (refresh) rc[p] = drift_block(k0, k1, tid, ogen, opop, p, g);
         if (Nc <= 0) { nx[p] = 0.0f; continue; }
         float im = 0.0f;
 #pragma unroll
@@ -99,10 +113,9 @@ __device__ __forceinline__ void generation_update(float *x, const GenRow<P> &R, 
         const float N = R.fN[p];
         const float mean = xs * N;
         const float var = (1.0f - xs) * mean;
-        if (refresh) rc[p] = drift_block(k0, k1, tid, ogen, opop, p, g);
         int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N);
         j = j < 0 ? 0 : (j > Nc ? Nc : j);
-        nx[p] = (float)j / N;
+        nx[p] = (float)((double)(float)j * R.invN[p]);   /* == (float)j / N */
     }
 #pragma unroll
     for (int p = 0; p < P; p++) x[p] = nx[p];

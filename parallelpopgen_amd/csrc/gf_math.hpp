@@ -226,20 +226,38 @@ GF_HD int exit_index(float mean) {
     return kExitIndex[k];
 }
 
+/* true if `b` holds for any active lane of the calling wave (device), or
+ * for the single calling thread (host) */
+GF_HD bool wave_any(bool b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(b) != 0;
+#else
+    return b;
+#endif
+}
+
+/* Poisson CDF inversion truncated as in the reference (shared.cuh:168-249):
+ * the first j with CDF(j) >= r, or the early-exit index of `mean` if that
+ * comes first, or 70.  On the device the loop runs in step across the wave
+ * while any lane is still searching, and a lane that has found its j keeps
+ * it (the terms it computes after that are ignored); this returns the same
+ * j as a per-lane early return without one nested branch per term. */
 GF_HD int poiscdfinv(float r, float mean) {
     float emu = exp_f(-1.0f * mean);
     float cdf = emu;
-    if (cdf >= r) return 0;
     const int jstop = exit_index(mean);
-    int j = 1;
-#pragma unroll
+    bool done = cdf >= r;
+    int k = 0;
+#pragma unroll 2
     for (int jj = 1; jj < 70; jj++) {
+        if (!wave_any(!done)) break;
         emu = emu * (mean * kRecip[jj - 1]);
         cdf = cdf + emu;
-        j = jj;
-        if (cdf >= r || jj == jstop) return j;
+        const bool hit = !done && ((cdf >= r) || (jj == jstop));
+        k = hit ? jj : k;
+        done = done || hit;
     }
-    return 70;
+    return done ? k : 70;
 }
 
 /* binomial CDF inversion (N < 100 only): kept out of line, it is off the

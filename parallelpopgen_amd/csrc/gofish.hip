@@ -43,7 +43,7 @@ void set_error(const std::string &m) { t_err = m; }
 const std::string &get_error() { return t_err; }
 static bool g_timing = false;
 bool timing_enabled() { return g_timing; }
-static int g_kpl = 32;
+static int g_kpl = 16;   /* generations per advance launch (1..KMAX) */
 int generations_per_launch() { return g_kpl; }
 
 int select_device(int device) {
@@ -196,6 +196,7 @@ struct PlainRun {
             for (int p = 0; p < P; p++) {
                 r.Nc[p] = Nc[p];
                 r.fN[p] = (float)Nc[p];
+                r.invN[p] = Nc[p] > 0 ? 1.0 / (double)(float)Nc[p] : 0.0;
                 const float F = pg[p].F, h = pg[p].h;
                 r.sa[p] = pg[p].sel_slope;
                 r.sb[p] = pg[p].sel_intercept;

@@ -1,0 +1,572 @@
+/*
+ * surrogate.hip -- learned surrogate of the simulator's sample SFS
+ * (no reference counterpart).
+ *
+ * An ensemble of E multilayer perceptrons, each
+ *     6 inputs (padded to 64) -> 4 hidden layers of 512 (GELU) -> 100 outputs
+ * (99 softmax logits for the n = 100 SFS bins 1..99, one log-total head),
+ * BF16 weights and activations, FP32 accumulation, trained by this file's
+ * forward, backward and Adam kernels.  Every matrix product runs on the
+ * gfx950 matrix cores (v_mfma_f32_32x32x16_bf16) through one tiled kernel
+ * whose global->LDS stage handles the three operand layouts of
+ * forward (X W^T), input gradient (dZ W) and weight gradient (dZ^T X);
+ * the ensemble members are the grid's z dimension.
+ */
+#include "gf_runtime.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace gf {
+namespace sg {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int IN_PAD = 64;    /* input features padded to one K tile */
+constexpr int HID = 512;
+constexpr int NHID = 4;
+constexpr int OUT = 100;      /* 99 bins + log-total */
+constexpr int OUT_PAD = 128;
+constexpr int NBINS = 99;
+constexpr int TILE = 64;
+constexpr int KSTEP = 32;
+constexpr int LDSK = KSTEP + 8; /* padded LDS row (bf16 elements) */
+
+/* parameter layout of one model, in floats */
+struct Layout {
+    size_t W[NHID + 1], b[NHID + 1];
+    int in[NHID + 1], out[NHID + 1];
+    size_t total;
+    Layout() {
+        size_t o = 0;
+        for (int l = 0; l <= NHID; l++) {
+            in[l] = l == 0 ? IN_PAD : HID;
+            out[l] = l == NHID ? OUT_PAD : HID;
+            W[l] = o; o += (size_t)in[l] * out[l];
+            b[l] = o; o += out[l];
+        }
+        total = o;
+    }
+};
+
+__device__ __forceinline__ float gelu(float x) {
+    const float c = 0.7978845608028654f;   /* sqrt(2/pi) */
+    return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
+}
+
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float c = 0.7978845608028654f;
+    const float u = c * (x + 0.044715f * x * x * x);
+    const float t = tanhf(u);
+    const float du = c * (1.0f + 3.0f * 0.044715f * x * x);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
+}
+
+enum Epi { EPI_HIDDEN = 0, EPI_OUTPUT = 1, EPI_DX = 2, EPI_DW = 3 };
+
+struct GemmArgs {
+    const bf16 *A; int64_t lda; int a_km; int64_t sA;   /* A(m,k) = a_km ? A[k*lda+m] : A[m*lda+k] */
+    const bf16 *B; int64_t ldb; int b_nk; int64_t sB;   /* B(k,n) = b_nk ? B[n*ldb+k] : B[k*ldb+n] */
+    int M, N, K;
+    /* epilogue */
+    const float *bias; int64_t sBias;
+    float *Cf; int64_t ldc; int64_t sCf;                 /* fp32 output (Z, logits, dW) */
+    bf16 *Cb; int64_t sCb;                               /* bf16 output (H or dZ) */
+    const float *Zprev; int64_t sZ;                      /* pre-activation for GELU' */
+    int epi;
+};
+
+/* C = A B (+ epilogue), 64x64 tile per workgroup of 4 waves; each wave owns
+ * a 32x32 sub-tile accumulated by 32x32x16 BF16 MFMAs from LDS. */
+__global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
+    __shared__ bf16 As[TILE][LDSK];   /* [m][k] */
+    __shared__ bf16 Bs[TILE][LDSK];   /* [n][k] */
+    const int z = blockIdx.z;
+    const bf16 *A = g.A + z * g.sA;
+    const bf16 *B = g.B + z * g.sB;
+    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < g.K; k0 += KSTEP) {
+        /* ---- global -> LDS: 64 x 32 elements of each operand ---- */
+        if (!g.a_km) {
+            const int m = t >> 2, kc = (t & 3) * 8;
+            const bf16x8 v = *(const bf16x8 *)(A + (int64_t)(m0 + m) * g.lda + k0 + kc);
+            *(bf16x8 *)&As[m][kc] = v;
+        } else {
+            const int k = t >> 3, mc = (t & 7) * 8;
+            const bf16x8 v = *(const bf16x8 *)(A + (int64_t)(k0 + k) * g.lda + m0 + mc);
+#pragma unroll
+            for (int q = 0; q < 8; q++) As[mc + q][k] = v[q];
+        }
+        if (g.b_nk) {
+            const int n = t >> 2, kc = (t & 3) * 8;
+            const bf16x8 v = *(const bf16x8 *)(B + (int64_t)(n0 + n) * g.ldb + k0 + kc);
+            *(bf16x8 *)&Bs[n][kc] = v;
+        } else {
+            const int k = t >> 3, nc = (t & 7) * 8;
+            const bf16x8 v = *(const bf16x8 *)(B + (int64_t)(k0 + k) * g.ldb + n0 + nc);
+#pragma unroll
+            for (int q = 0; q < 8; q++) Bs[nc + q][k] = v[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KSTEP; kk += 16) {
+            const bf16x8 a = *(const bf16x8 *)&As[wm * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+            const bf16x8 b = *(const bf16x8 *)&Bs[wn * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    /* ---- epilogue: accumulator r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32] ---- */
+    const int n = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t o = (int64_t)m * g.ldc + n;
+        const float v = acc[r];
+        if (g.epi == EPI_HIDDEN) {
+            const float zz = v + g.bias[z * g.sBias + n];
+            g.Cf[z * g.sCf + o] = zz;
+            g.Cb[z * g.sCb + o] = (bf16)gelu(zz);
+        } else if (g.epi == EPI_OUTPUT) {
+            g.Cf[z * g.sCf + o] = v + g.bias[z * g.sBias + n];
+        } else if (g.epi == EPI_DX) {
+            g.Cb[z * g.sCb + o] = (bf16)(v * gelu_grad(g.Zprev[z * g.sZ + o]));
+        } else {
+            g.Cf[z * g.sCf + o] = v;
+        }
+    }
+}
+
+/* softmax cross-entropy on the 99 bins + weighted squared error on the
+ * log-total head; writes dLogits (bf16, /batch) and the summed loss */
+__global__ void k_loss(const float *__restrict__ logits, int64_t sL, const float *__restrict__ y,
+                       const float *__restrict__ tot, int B, float w_tot, bf16 *__restrict__ dlog, int64_t sD,
+                       float *__restrict__ loss) {
+    const int z = blockIdx.y;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float *L = logits + z * sL + (int64_t)b * OUT_PAD;
+    bf16 *D = dlog + z * sD + (int64_t)b * OUT_PAD;
+    const float *Y = y + (int64_t)b * NBINS;
+    float mx = L[0];
+    for (int k = 1; k < NBINS; k++) mx = fmaxf(mx, L[k]);
+    float s = 0.f;
+    for (int k = 0; k < NBINS; k++) s += __expf(L[k] - mx);
+    const float ls = logf(s) + mx;
+    float ce = 0.f;
+    const float invB = 1.0f / (float)B;
+    for (int k = 0; k < NBINS; k++) {
+        const float p = __expf(L[k] - ls);
+        ce -= Y[k] * (L[k] - ls);
+        D[k] = (bf16)((p - Y[k]) * invB);
+    }
+    const float e = L[NBINS] - tot[b];
+    D[NBINS] = (bf16)(2.0f * w_tot * e * invB);
+    for (int k = OUT; k < OUT_PAD; k++) D[k] = (bf16)0.0f;
+    atomicAdd(&loss[z], (ce + w_tot * e * e) * invB);
+}
+
+/* column sums of a bf16 [B][N] matrix -> fp32 [N] (bias gradients) */
+__global__ void k_colsum(const bf16 *__restrict__ dz, int64_t sD, int B, int N, float *__restrict__ out, int64_t sO) {
+    const int z = blockIdx.y;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int b = 0; b < B; b++) s += (float)dz[z * sD + (int64_t)b * N + n];
+    out[z * sO + n] = s;
+}
+
+/* Adam on fp32 master weights; refreshes the bf16 copy */
+__global__ void k_adam(float *__restrict__ w, bf16 *__restrict__ wb, const float *__restrict__ gr,
+                       float *__restrict__ m, float *__restrict__ v, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float g = gr[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * g;
+        const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        wb[i] = (bf16)wi;
+    }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ in, bf16 *__restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (bf16)in[i];
+}
+
+/* rows ord[0..B) of the staged training set -> the batch buffers */
+__global__ void k_gather(const float *__restrict__ X, const float *__restrict__ Y, const float *__restrict__ T,
+                         const int32_t *__restrict__ ord, int B, float *__restrict__ x, float *__restrict__ y,
+                         float *__restrict__ t) {
+    const int b = blockIdx.x;
+    const int64_t src = ord[b];
+    for (int k = threadIdx.x; k < NBINS; k += blockDim.x) y[(int64_t)b * NBINS + k] = Y[src * NBINS + k];
+    if (threadIdx.x < 6) x[(int64_t)b * 6 + threadIdx.x] = X[src * 6 + threadIdx.x];
+    if (threadIdx.x == 0) t[b] = T[src];
+}
+
+/* inputs [B][6] fp32 -> [B][64] bf16, zero padded */
+__global__ void k_pack_inputs(const float *__restrict__ x, int B, int nin, bf16 *__restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * IN_PAD) return;
+    const int b = (int)(i / IN_PAD), k = (int)(i % IN_PAD);
+    out[i] = (bf16)(k < nin ? x[(int64_t)b * nin + k] : 0.0f);
+}
+
+/* ensemble statistics of the predictions: per-model softmax, mean SFS,
+ * mean log-total and the ensemble disagreement (largest relative standard
+ * deviation across models over the bins with mean mass >= 1e-3) */
+__global__ void k_predict_stats(const float *__restrict__ logits, int64_t sL, int E, int B,
+                                float *__restrict__ sfs, float *__restrict__ tot, float *__restrict__ dis) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float mean[NBINS], m2[NBINS];
+    for (int k = 0; k < NBINS; k++) { mean[k] = 0.f; m2[k] = 0.f; }
+    float tsum = 0.f;
+    for (int z = 0; z < E; z++) {
+        const float *L = logits + z * sL + (int64_t)b * OUT_PAD;
+        float mx = L[0];
+        for (int k = 1; k < NBINS; k++) mx = fmaxf(mx, L[k]);
+        float s = 0.f;
+        for (int k = 0; k < NBINS; k++) s += __expf(L[k] - mx);
+        for (int k = 0; k < NBINS; k++) {
+            const float p = __expf(L[k] - mx) / s;
+            mean[k] += p;
+            m2[k] += p * p;
+        }
+        tsum += L[NBINS];
+    }
+    float worst = 0.f;
+    for (int k = 0; k < NBINS; k++) {
+        const float mu = mean[k] / E;
+        const float var = fmaxf(m2[k] / E - mu * mu, 0.f);
+        sfs[(int64_t)b * NBINS + k] = mu;
+        if (mu >= 1e-3f) worst = fmaxf(worst, sqrtf(var) / mu);
+    }
+    tot[b] = tsum / E;
+    dis[b] = worst;
+}
+
+} /* namespace sg */
+} /* namespace gf */
+
+using namespace gf;
+using namespace gf::sg;
+
+struct gf_sg_model {
+    int E = 4, Bmax = 0;
+    Layout lay;
+    hipStream_t st = nullptr;
+    DevBuf<float> w, g, m, v;       /* [E][total] */
+    DevBuf<bf16> wb;
+    DevBuf<bf16> x;                 /* [B][64] */
+    DevBuf<float> Z;                /* [E][NHID][B][512] */
+    DevBuf<bf16> H;                 /* [E][NHID][B][512] */
+    DevBuf<float> logits;           /* [E][B][128] */
+    DevBuf<bf16> dlog;              /* [E][B][128] */
+    DevBuf<bf16> dz0, dz1;          /* [E][B][512] */
+    DevBuf<float> y, tot, loss, xin;
+    DevBuf<float> psfs, ptot, pdis;
+    int64_t step = 0;
+    float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    ~gf_sg_model() { if (st) (void)hipStreamDestroy(st); }
+};
+
+namespace gf {
+namespace sg {
+
+static int gemm(gf_sg_model &S, GemmArgs a) {
+    if (a.M % TILE || a.N % TILE || a.K % KSTEP) { set_error("surrogate gemm: shape not a multiple of the tile"); return GF_ERR_ARG; }
+    hipLaunchKernelGGL(k_gemm, dim3(a.N / TILE, a.M / TILE, S.E), dim3(256), 0, S.st, a);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+static int ensure_batch(gf_sg_model &S, int B) {
+    if (B <= S.Bmax) return GF_OK;
+    const int E = S.E;
+    GF_TRY(S.x.alloc((size_t)B * IN_PAD));
+    GF_TRY(S.Z.alloc((size_t)E * NHID * B * HID));
+    GF_TRY(S.H.alloc((size_t)E * NHID * B * HID));
+    GF_TRY(S.logits.alloc((size_t)E * B * OUT_PAD));
+    GF_TRY(S.dlog.alloc((size_t)E * B * OUT_PAD));
+    GF_TRY(S.dz0.alloc((size_t)E * B * HID));
+    GF_TRY(S.dz1.alloc((size_t)E * B * HID));
+    GF_TRY(S.y.alloc((size_t)B * NBINS));
+    GF_TRY(S.tot.alloc(B));
+    GF_TRY(S.xin.alloc((size_t)B * 6));
+    GF_TRY(S.psfs.alloc((size_t)B * NBINS));
+    GF_TRY(S.ptot.alloc(B));
+    GF_TRY(S.pdis.alloc(B));
+    S.Bmax = B;
+    return GF_OK;
+}
+
+/* forward pass of the whole ensemble on S.x (B rows) */
+static int forward(gf_sg_model &S, int B) {
+    const Layout &L = S.lay;
+    const int64_t sW = (int64_t)L.total;
+    const int64_t sAct = (int64_t)NHID * B * HID;
+    for (int l = 0; l <= NHID; l++) {
+        GemmArgs a{};
+        if (l == 0) { a.A = S.x.p; a.lda = IN_PAD; a.sA = 0; }
+        else { a.A = S.H.p + (int64_t)(l - 1) * B * HID; a.lda = HID; a.sA = sAct; }
+        a.a_km = 0;
+        a.B = S.wb.p + L.W[l]; a.ldb = L.in[l]; a.b_nk = 1; a.sB = sW;
+        a.M = B; a.N = L.out[l]; a.K = L.in[l];
+        a.bias = S.w.p + L.b[l]; a.sBias = sW;
+        if (l < NHID) {
+            a.epi = EPI_HIDDEN;
+            a.Cf = S.Z.p + (int64_t)l * B * HID; a.sCf = sAct; a.ldc = HID;
+            a.Cb = S.H.p + (int64_t)l * B * HID; a.sCb = sAct;
+        } else {
+            a.epi = EPI_OUTPUT;
+            a.Cf = S.logits.p; a.sCf = (int64_t)B * OUT_PAD; a.ldc = OUT_PAD;
+        }
+        GF_TRY(gemm(S, a));
+    }
+    return GF_OK;
+}
+
+/* loss + backward + Adam for one batch already in S.x / S.y / S.tot */
+static int train_step(gf_sg_model &S, int B, float lr, float w_tot, bool update) {
+    const Layout &L = S.lay;
+    const int E = S.E;
+    const int64_t sW = (int64_t)L.total;
+    const int64_t sAct = (int64_t)NHID * B * HID;
+    GF_TRY(forward(S, B));
+    hipLaunchKernelGGL(k_loss, dim3((B + 127) / 128, E), dim3(128), 0, S.st, S.logits.p, (int64_t)B * OUT_PAD,
+                       S.y.p, S.tot.p, B, w_tot, S.dlog.p, (int64_t)B * OUT_PAD, S.loss.p);
+    GF_HIP(hipGetLastError());
+    /* dY of layer l lives in `dcur` ([E][B][out_l] bf16) */
+    const bf16 *dcur = S.dlog.p;
+    int64_t sD = (int64_t)B * OUT_PAD;
+    for (int l = NHID; l >= 0; l--) {
+        const int O = L.out[l], I = L.in[l];
+        /* dW_l = dY^T X_l : M = O, N = I, K = B */
+        GemmArgs a{};
+        a.A = dcur; a.lda = O; a.a_km = 1; a.sA = sD;
+        if (l == 0) { a.B = S.x.p; a.ldb = IN_PAD; a.sB = 0; }
+        else { a.B = S.H.p + (int64_t)(l - 1) * B * HID; a.ldb = HID; a.sB = sAct; }
+        a.b_nk = 0;
+        a.M = O; a.N = I; a.K = B;
+        a.epi = EPI_DW; a.Cf = S.g.p + L.W[l]; a.ldc = I; a.sCf = sW;
+        GF_TRY(gemm(S, a));
+        hipLaunchKernelGGL(k_colsum, dim3((O + 127) / 128, E), dim3(128), 0, S.st, dcur, sD, B, O, S.g.p + L.b[l], sW);
+        GF_HIP(hipGetLastError());
+        if (l == 0) break;
+        /* dZ_(l-1) = (dY W_l) * gelu'(Z_(l-1)) : M = B, N = I, K = O */
+        bf16 *dnext = (dcur == S.dz0.p) ? S.dz1.p : S.dz0.p;
+        GemmArgs b{};
+        b.A = dcur; b.lda = O; b.a_km = 0; b.sA = sD;
+        b.B = S.wb.p + L.W[l]; b.ldb = I; b.b_nk = 0; b.sB = sW;
+        b.M = B; b.N = I; b.K = O;
+        b.epi = EPI_DX; b.Cb = dnext; b.sCb = (int64_t)B * HID; b.ldc = HID;
+        b.Zprev = S.Z.p + (int64_t)(l - 1) * B * HID; b.sZ = sAct;
+        GF_TRY(gemm(S, b));
+        dcur = dnext;
+        sD = (int64_t)B * HID;
+    }
+    if (update) {
+        S.step++;
+        const float c1 = 1.0f - std::pow(S.b1, (float)S.step);
+        const float c2 = 1.0f - std::pow(S.b2, (float)S.step);
+        const int64_t n = (int64_t)E * L.total;
+        hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, S.st, S.w.p, S.wb.p, S.g.p, S.m.p, S.v.p, n, lr, S.b1,
+                           S.b2, S.eps, c1, c2);
+        GF_HIP(hipGetLastError());
+    }
+    return GF_OK;
+}
+
+static int load_batch(gf_sg_model &S, const float *x, const float *y, const float *t, int B) {
+    GF_HIP(hipMemcpyAsync(S.xin.p, x, (size_t)B * 6 * sizeof(float), hipMemcpyHostToDevice, S.st));
+    if (y) GF_HIP(hipMemcpyAsync(S.y.p, y, (size_t)B * NBINS * sizeof(float), hipMemcpyHostToDevice, S.st));
+    if (t) GF_HIP(hipMemcpyAsync(S.tot.p, t, (size_t)B * sizeof(float), hipMemcpyHostToDevice, S.st));
+    hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S.st, S.xin.p, B, 6, S.x.p);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
+}
+
+} /* namespace sg */
+} /* namespace gf */
+
+extern "C" {
+
+int32_t gf_sg_create(int32_t ensemble, uint64_t seed, int32_t device, gf_sg_model **out) {
+    if (!out || ensemble < 1 || ensemble > 16) { set_error("gf_sg_create: invalid arguments"); return GF_ERR_ARG; }
+    *out = nullptr;
+    GF_TRY(select_device(device));
+    std::unique_ptr<gf_sg_model> S(new gf_sg_model);
+    S->E = ensemble;
+    GF_HIP(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+    const size_t n = (size_t)S->E * S->lay.total;
+    GF_TRY(S->w.alloc(n));
+    GF_TRY(S->g.alloc(n));
+    GF_TRY(S->m.alloc(n));
+    GF_TRY(S->v.alloc(n));
+    GF_TRY(S->wb.alloc(n));
+    GF_TRY(S->loss.alloc(S->E));
+    /* He-style initialisation from a seeded host generator: W ~ U(-a, a),
+     * a = sqrt(6 / fan_in); biases 0; padded input columns and output rows 0 */
+    std::vector<float> h(n, 0.0f);
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto next = [&st]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (float)((st >> 40) & 0xffffff) / 16777216.0f;
+    };
+    for (int e = 0; e < S->E; e++) {
+        for (int l = 0; l <= NHID; l++) {
+            const int I = S->lay.in[l], O = S->lay.out[l];
+            const int real_in = l == 0 ? 6 : I, real_out = l == NHID ? OUT : O;
+            const float a = std::sqrt(6.0f / (float)real_in);
+            for (int o = 0; o < O; o++)
+                for (int i = 0; i < I; i++) {
+                    float v = (2.0f * next() - 1.0f) * a;
+                    if (i >= real_in || o >= real_out) v = 0.0f;
+                    h[(size_t)e * S->lay.total + S->lay.W[l] + (size_t)o * I + i] = v;
+                }
+        }
+    }
+    GF_HIP(hipMemcpy(S->w.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    GF_HIP(hipMemset(S->m.p, 0, n * sizeof(float)));
+    GF_HIP(hipMemset(S->v.p, 0, n * sizeof(float)));
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, S->st, S->w.p, S->wb.p, (int64_t)n);
+    GF_HIP(hipGetLastError());
+    GF_HIP(hipStreamSynchronize(S->st));
+    *out = S.release();
+    return GF_OK;
+}
+
+void gf_sg_free(gf_sg_model *S) { delete S; }
+
+int32_t gf_sg_num_params(const gf_sg_model *S) { return S ? (int32_t)S->lay.total : GF_ERR_ARG; }
+
+int32_t gf_sg_get_params(gf_sg_model *S, float *out) {
+    if (!S || !out) return GF_ERR_ARG;
+    GF_HIP(hipStreamSynchronize(S->st));
+    GF_HIP(hipMemcpy(out, S->w.p, (size_t)S->E * S->lay.total * sizeof(float), hipMemcpyDeviceToHost));
+    return GF_OK;
+}
+
+int32_t gf_sg_set_params(gf_sg_model *S, const float *in) {
+    if (!S || !in) return GF_ERR_ARG;
+    const size_t n = (size_t)S->E * S->lay.total;
+    GF_HIP(hipMemcpy(S->w.p, in, n * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, S->st, S->w.p, S->wb.p, (int64_t)n);
+    GF_HIP(hipGetLastError());
+    GF_HIP(hipStreamSynchronize(S->st));
+    return GF_OK;
+}
+
+/* one forward + backward on a batch (B a multiple of 64) without updating:
+ * gradients [E][total] and per-model loss, for gradient checks */
+int32_t gf_sg_gradients(gf_sg_model *S, const float *x, const float *y, const float *t, int32_t B, float w_tot,
+                        float *grad_out, float *loss_out) {
+    if (!S || !x || !y || !t || B <= 0 || B % TILE) { set_error("gf_sg_gradients: invalid arguments"); return GF_ERR_ARG; }
+    GF_TRY(ensure_batch(*S, B));
+    GF_HIP(hipMemsetAsync(S->loss.p, 0, S->E * sizeof(float), S->st));
+    GF_TRY(load_batch(*S, x, y, t, B));
+    GF_TRY(train_step(*S, B, 0.0f, w_tot, false));
+    GF_HIP(hipStreamSynchronize(S->st));
+    if (grad_out) GF_HIP(hipMemcpy(grad_out, S->g.p, (size_t)S->E * S->lay.total * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss_out) GF_HIP(hipMemcpy(loss_out, S->loss.p, S->E * sizeof(float), hipMemcpyDeviceToHost));
+    return GF_OK;
+}
+
+/* epochs over n examples (host arrays), mini-batches of B drawn in the
+ * order given by `order` (n ints per epoch, or NULL for 0..n-1); returns
+ * the mean loss per epoch (averaged over models) */
+int32_t gf_sg_train(gf_sg_model *S, const float *x, const float *y, const float *t, int32_t n, int32_t epochs,
+                    int32_t B, float lr, float w_tot, const int32_t *order, float *epoch_loss) {
+    if (!S || !x || !y || !t || n < B || B <= 0 || B % TILE || epochs <= 0) { set_error("gf_sg_train: invalid arguments"); return GF_ERR_ARG; }
+    GF_TRY(ensure_batch(*S, B));
+    /* stage the whole training set on the device once */
+    DevBuf<float> dx, dy, dt;
+    DevBuf<int32_t> dord;
+    GF_TRY(dx.alloc((size_t)n * 6));
+    GF_TRY(dy.alloc((size_t)n * NBINS));
+    GF_TRY(dt.alloc(n));
+    GF_HIP(hipMemcpyAsync(dx.p, x, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice, S->st));
+    GF_HIP(hipMemcpyAsync(dy.p, y, (size_t)n * NBINS * sizeof(float), hipMemcpyHostToDevice, S->st));
+    GF_HIP(hipMemcpyAsync(dt.p, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice, S->st));
+    const int nb = n / B;
+    std::vector<float> lossv(S->E);
+    if (order) {
+        GF_TRY(dord.alloc((size_t)epochs * n));
+        GF_HIP(hipMemcpyAsync(dord.p, order, (size_t)epochs * n * sizeof(int32_t), hipMemcpyHostToDevice, S->st));
+    }
+    for (int e = 0; e < epochs; e++) {
+        GF_HIP(hipMemsetAsync(S->loss.p, 0, S->E * sizeof(float), S->st));
+        for (int b = 0; b < nb; b++) {
+            if (order) {
+                hipLaunchKernelGGL(k_gather, dim3(B), dim3(128), 0, S->st, dx.p, dy.p, dt.p,
+                                   dord.p + (size_t)e * n + (size_t)b * B, B, S->xin.p, S->y.p, S->tot.p);
+                GF_HIP(hipGetLastError());
+            } else {
+                GF_HIP(hipMemcpyAsync(S->xin.p, dx.p + (size_t)b * B * 6, (size_t)B * 6 * sizeof(float), hipMemcpyDeviceToDevice, S->st));
+                GF_HIP(hipMemcpyAsync(S->y.p, dy.p + (size_t)b * B * NBINS, (size_t)B * NBINS * sizeof(float), hipMemcpyDeviceToDevice, S->st));
+                GF_HIP(hipMemcpyAsync(S->tot.p, dt.p + (size_t)b * B, (size_t)B * sizeof(float), hipMemcpyDeviceToDevice, S->st));
+            }
+            hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S->st, S->xin.p, B, 6, S->x.p);
+            GF_TRY(train_step(*S, B, lr, w_tot, true));
+        }
+        GF_HIP(hipMemcpyAsync(lossv.data(), S->loss.p, S->E * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        GF_HIP(hipStreamSynchronize(S->st));
+        if (epoch_loss) {
+            float s = 0.f;
+            for (float v : lossv) s += v;
+            epoch_loss[e] = s / S->E / nb;
+        }
+    }
+    GF_HIP(hipStreamSynchronize(S->st));
+    return GF_OK;
+}
+
+/* ensemble prediction for n inputs: mean normalised SFS (bins 1..99),
+ * mean log-total and the ensemble disagreement per input */
+int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, float *tot, float *dis) {
+    if (!S || !x || n <= 0 || !sfs) { set_error("gf_sg_predict: invalid arguments"); return GF_ERR_ARG; }
+    const int B = std::min(8192, ((n + TILE - 1) / TILE) * TILE);
+    GF_TRY(ensure_batch(*S, B));
+    std::vector<float> xb((size_t)B * 6, 0.0f);
+    for (int off = 0; off < n; off += B) {
+        const int cnt = std::min(B, n - off);
+        memcpy(xb.data(), x + (size_t)off * 6, (size_t)cnt * 6 * sizeof(float));
+        if (cnt < B) memset(xb.data() + (size_t)cnt * 6, 0, (size_t)(B - cnt) * 6 * sizeof(float));
+        GF_TRY(load_batch(*S, xb.data(), nullptr, nullptr, B));
+        GF_TRY(forward(*S, B));
+        hipLaunchKernelGGL(k_predict_stats, dim3((B + 63) / 64), dim3(64), 0, S->st, S->logits.p, (int64_t)B * OUT_PAD,
+                           S->E, B, S->psfs.p, S->ptot.p, S->pdis.p);
+        GF_HIP(hipGetLastError());
+        GF_HIP(hipMemcpyAsync(sfs + (size_t)off * NBINS, S->psfs.p, (size_t)cnt * NBINS * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        if (tot) GF_HIP(hipMemcpyAsync(tot + off, S->ptot.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        if (dis) GF_HIP(hipMemcpyAsync(dis + off, S->pdis.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        GF_HIP(hipStreamSynchronize(S->st));
+    }
+    return GF_OK;
+}
+
+/* raw ensemble logits for n inputs (n a multiple of 64): [E][n][128] */
+int32_t gf_sg_forward(gf_sg_model *S, const float *x, int32_t n, float *logits) {
+    if (!S || !x || !logits || n <= 0 || n % TILE) { set_error("gf_sg_forward: invalid arguments"); return GF_ERR_ARG; }
+    GF_TRY(ensure_batch(*S, n));
+    GF_TRY(load_batch(*S, x, nullptr, nullptr, n));
+    GF_TRY(forward(*S, n));
+    GF_HIP(hipStreamSynchronize(S->st));
+    for (int e = 0; e < S->E; e++)
+        GF_HIP(hipMemcpy(logits + (size_t)e * n * OUT_PAD, S->logits.p + (size_t)e * n * OUT_PAD, (size_t)n * OUT_PAD * sizeof(float), hipMemcpyDeviceToHost));
+    return GF_OK;
+}
+
+} /* extern "C" */

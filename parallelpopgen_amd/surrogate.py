@@ -1,0 +1,212 @@
+"""Learned surrogate of the batched simulator's sample SFS.
+
+Parameter space (per draw): selection gamma ~ U(-50, 10) (2*N0*s/(1+F) at
+the ancestral size), dominance h in {0, 0.5, 1}, inbreeding F ~ U(0, 0.2),
+one instantaneous size change by a factor G ~ logU(1, 100) that lasted T ~
+logU(10, 2000) generations before sampling.  Output: the expected SFS of an
+n = 100 sample, as normalised bins 1..99 plus log(total / theta).
+
+Simulation scale.  Draws are simulated at an ancestral size of
+SIM_N0 = 1,000 diploids standing for REF_N0 = 10,000: generations are
+divided and the per-generation mutation rate multiplied by
+REF_N0 / SIM_N0 (the reference's population-scaling rule,
+examples/usage_and_examples.dox:86-102, under which the normalised SFS is
+invariant); gamma is kept.  theta = 4 * SIM_N0 * mu * L is fixed at
+THETA per simulation.
+
+Network: include/gofish_capi.h gf_sg_* (ensemble of MLPs trained on the
+GPU by the library's own kernels).  Inference falls back to simulation for
+inputs outside the training box or whose ensemble disagreement exceeds a
+threshold fixed on held-out data.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import capi
+
+REF_N0 = 10000
+SIM_N0 = 1000
+SCALE = REF_N0 // SIM_N0
+MU = 1e-9 * SCALE
+THETA = 200.0
+N_SAMPLE = 100
+NBINS = 99
+GAMMA_RANGE = (-50.0, 10.0)
+H_VALUES = (0.0, 0.5, 1.0)
+F_RANGE = (0.0, 0.2)
+GROWTH_RANGE = (1.0, 100.0)
+T_RANGE = (10.0, 2000.0)
+GAMMA_FLOOR = 1e-2
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Network:
+    """Ensemble of MLPs living on the GPU (C ABI gf_sg_*)."""
+
+    def __init__(self, ensemble=4, seed=1, device=-1):
+        self.lib = capi.load()
+        self.E = ensemble
+        h = C.c_void_p()
+        capi.check(self.lib.gf_sg_create(ensemble, seed, device, C.byref(h)))
+        self.h = h
+        self.n_params = self.lib.gf_sg_num_params(h)
+
+    def close(self):
+        if self.h:
+            self.lib.gf_sg_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.empty((self.E, self.n_params), np.float32)
+        capi.check(self.lib.gf_sg_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32).reshape(self.E, self.n_params)
+        capi.check(self.lib.gf_sg_set_params(self.h, _p(p)))
+
+    def gradients(self, x, y, t, w_tot=0.1):
+        x, y, t = (np.ascontiguousarray(a, np.float32) for a in (x, y, t))
+        g = np.empty((self.E, self.n_params), np.float32)
+        loss = np.empty(self.E, np.float32)
+        capi.check(self.lib.gf_sg_gradients(self.h, _p(x), _p(y), _p(t), len(x), w_tot, _p(g), _p(loss)))
+        return loss, g
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty((self.E, len(x), 128), np.float32)
+        capi.check(self.lib.gf_sg_forward(self.h, _p(x), len(x), _p(out)))
+        return out
+
+    def train(self, x, y, t, epochs=10, batch=512, lr=1e-3, w_tot=0.1, seed=0, shuffle=True):
+        """`epochs` passes of Adam over mini-batches of `batch` rows (a
+        multiple of 64); rows are visited in a fresh random order each
+        epoch, or in storage order when shuffle is False.  Returns the mean
+        training loss of every epoch."""
+        x, y, t = (np.ascontiguousarray(a, np.float32) for a in (x, y, t))
+        order = None
+        if shuffle:
+            rng = np.random.default_rng(seed)
+            order = np.ascontiguousarray(np.stack([rng.permutation(len(x)) for _ in range(epochs)]), np.int32)
+        losses = np.empty(epochs, np.float32)
+        capi.check(self.lib.gf_sg_train(self.h, _p(x), _p(y), _p(t), len(x), epochs, batch, lr, w_tot,
+                                        _p(order), _p(losses)))
+        return losses
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n = len(x)
+        sfs = np.empty((n, NBINS), np.float32)
+        tot = np.empty(n, np.float32)
+        dis = np.empty(n, np.float32)
+        capi.check(self.lib.gf_sg_predict(self.h, _p(x), n, _p(sfs), _p(tot), _p(dis)))
+        return sfs, tot, dis
+
+
+# ---------------- parameter draws, features, simulation ----------------
+
+def draw(n, rng):
+    return dict(
+        gamma=rng.uniform(*GAMMA_RANGE, n),
+        h=rng.choice(H_VALUES, n),
+        F=rng.uniform(*F_RANGE, n),
+        growth=np.exp(rng.uniform(math.log(GROWTH_RANGE[0]), math.log(GROWTH_RANGE[1]), n)),
+        T=np.exp(rng.uniform(math.log(T_RANGE[0]), math.log(T_RANGE[1]), n)),
+    )
+
+
+def features(p):
+    g = np.maximum(np.abs(p["gamma"]), GAMMA_FLOOR)
+    return np.stack([np.log(p["growth"]) / math.log(GROWTH_RANGE[1]),
+                     np.log(p["T"] / T_RANGE[0]) / math.log(T_RANGE[1] / T_RANGE[0]),
+                     np.log10(g) / 2.0,
+                     np.sign(p["gamma"]),
+                     p["h"],
+                     p["F"] / F_RANGE[1]], axis=1).astype(np.float32)
+
+
+def in_hull(x):
+    """training box of the features (with the exact bounds of draw())"""
+    lo = np.array([0, 0, math.log10(GAMMA_FLOOR) / 2, -1, 0, 0], np.float32) - 1e-6
+    hi = np.array([1, 1, math.log10(max(abs(GAMMA_RANGE[0]), GAMMA_RANGE[1])) / 2, 1, 1, 1], np.float32) + 1e-6
+    return np.all((x >= lo) & (x <= hi), axis=1)
+
+
+def batch_sims(p, seeds):
+    n = len(p["gamma"])
+    sims = np.zeros(n, capi.BATCH_SIM_DTYPE)
+    Nc0 = (2.0 * SIM_N0) / (1.0 + p["F"])   # Sim_Model effective-selection conversion
+    sims["N"] = SIM_N0
+    sims["N1"] = np.maximum(1, np.round(SIM_N0 * p["growth"])).astype(np.int32)
+    sims["F"] = p["F"]
+    sims["h"] = p["h"]
+    sims["mu"] = MU
+    sims["s"] = (p["gamma"] / Nc0).astype(np.float32)
+    sims["num_sites"] = THETA / (4.0 * SIM_N0 * MU)
+    sims["seed1"] = seeds.astype(np.int64) & 0x7FFFFFFF
+    sims["seed2"] = 0x5eed
+    sims["num_generations"] = np.maximum(1, np.round(p["T"] / SCALE)).astype(np.int32)
+    return sims
+
+
+def simulate(p, seeds, chunk=8192, compact_interval=16):
+    """expected n=100 sample SFS (bins 1..99) of every draw, by the batched simulator"""
+    n = len(seeds)
+    out = np.empty((n, NBINS), np.float64)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        sub = {k: v[a:b] for k, v in p.items()}
+        r = capi.batch_run(batch_sims(sub, seeds[a:b]), compact_interval=compact_interval, sfs_n=N_SAMPLE)
+        out[a:b] = r["sfs"][:, 1:]
+    return out
+
+
+def targets(sfs):
+    tot = np.maximum(sfs.sum(1), 1e-6)
+    y = (sfs / tot[:, None]).astype(np.float32)
+    t = np.log(tot / THETA).astype(np.float32)
+    return y, t
+
+
+def expected_counts(sfs_norm, log_tot):
+    return sfs_norm * (np.exp(log_tot) * THETA)[:, None]
+
+
+def poisson_deviance(obs, mu):
+    """mean per-bin Poisson deviance of observed (expected-SFS) counts
+    against means mu, per draw"""
+    obs = np.maximum(obs, 0.0)
+    mu = np.maximum(mu, 1e-12)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        term = np.where(obs > 0, obs * np.log(obs / mu), 0.0)
+    return (2.0 * (term - (obs - mu))).mean(1)
+
+
+class Surrogate:
+    """Network + fallback to simulation (out-of-hull inputs or ensemble
+    disagreement above `threshold`)."""
+
+    def __init__(self, net, threshold):
+        self.net = net
+        self.threshold = float(threshold)
+
+    def predict(self, p, seeds):
+        x = features(p)
+        sfs, tot, dis = self.net.predict(x)
+        counts = expected_counts(sfs, tot)
+        fallback = (~in_hull(x)) | (dis > self.threshold)
+        if fallback.any():
+            idx = np.nonzero(fallback)[0]
+            sub = {k: v[idx] for k, v in p.items()}
+            counts[idx] = simulate(sub, seeds[idx])
+        return counts, fallback

@@ -73,3 +73,23 @@ def test_batch_live_count_matches_plain(gpu):
         r = gpu.run(plain_desc(sim, 35))
         tot += int(r["live"][1:].sum())
     assert b["live_mutation_generations"] == tot
+
+
+def test_batch_size_change_matches_plain(gpu):
+    """N1 != N: the batch draws the starting state at N and simulates every
+    generation from 1 on at N1 (an instantaneous size change), exactly as
+    gf_run does with the size table N(0) = N, N(g >= 1) = N1."""
+    rng = np.random.default_rng(24)
+    sims = make_batch(12, rng, G_scale=0.5)
+    sims["N1"] = np.maximum(1, (sims["N"] * np.exp(rng.uniform(-1.5, 2.5, len(sims))))).astype(np.int32)
+    sims["N1"][0] = sims["N"][0]
+    b = gpu.batch_run(sims, compact_interval=8, sfs_n=40, keep_states=True)
+    for k, sim in enumerate(sims):
+        d = plain_desc(sim, 35)
+        d.params["N"][1:] = sim["N1"]
+        r = gpu.run(d)
+        last = r["samples"][-1]
+        f, ids = b["states"][k]
+        assert b["num_mutations"][k] == last["num_mutations"], k
+        assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), k
+        assert np.array_equal(ids, r["ids"]), k

@@ -41,7 +41,7 @@ def test_generations_per_launch_invariance(gpu, oracle, kpl):
     try:
         g = gpu.run(d)
     finally:
-        gpu.set_generations_per_launch(32)
+        gpu.set_generations_per_launch(16)
     assert_same(g, ref, "kpl=%d" % kpl)
 
 
@@ -85,14 +85,17 @@ def test_positive_selection_dominance_inbreeding(gpu, oracle):
     assert_same(gpu.run(d), oracle.run(d), "haploid")
 
 
-def split_scenario(G=400, L=5e7):
-    """population 1 splits off population 0 at generation 100, grows, then
-    both exchange migrants; population 0 goes extinct at generation 300."""
+def split_scenario(G=400, L=5e7, split=100):
+    """population 1 splits off population 0 at generation `split`, grows,
+    then both exchange migrants; population 0 goes extinct at generation
+    300.  A split generation that is not a multiple of 4 founds the
+    population inside a block of four generations that share one Philox
+    output."""
     P = 2
     params = np.zeros((G + 1, P), POP_GEN_DTYPE)
     g = np.arange(G + 1)
     params["N"][:, 0] = np.where(g < 300, 8000, 0)
-    params["N"][:, 1] = np.where(g < 100, 0, 2000 + 10 * (g - 100))
+    params["N"][:, 1] = np.where(g < split, 0, 2000 + 10 * (g - split))
     params["F"] = 0.0
     params["h"] = 0.5
     params["mu"] = 2e-9
@@ -101,20 +104,21 @@ def split_scenario(G=400, L=5e7):
     mig = np.zeros((G + 1, P, P), np.float32)
     mig[:, 0, 0] = 1
     mig[:, 1, 1] = 1
-    mig[100, 0, 1] = 1.0   # at the split pop 1 is made of pop 0 chromosomes
-    mig[100, 1, 1] = 0.0
+    mig[split, 0, 1] = 1.0   # at the split pop 1 is made of pop 0 chromosomes
+    mig[split, 1, 1] = 0.0
     m = np.float32(0.01)
-    for t in range(101, 300):
+    for t in range(split + 1, 300):
         mig[t] = [[1 - m, m], [m, 1 - m]]
     sample = np.zeros(G + 1, np.uint8)
     sample[[150, 299]] = 1
     return SimDesc(P, L, 0x1234, 0x5678, G, params, mig, None, sample, 25)
 
 
-def test_split_migration_extinction(gpu, oracle):
-    d = split_scenario()
+@pytest.mark.parametrize("split", [100, 101, 103])
+def test_split_migration_extinction(gpu, oracle, split):
+    d = split_scenario(split=split)
     g, o = gpu.run(d), oracle.run(d)
-    assert_same(g, o, "split")
+    assert_same(g, o, "split at %d" % split)
     assert g["samples"][-1]["extinct"] == [True, False]
 
 
