@@ -20,6 +20,15 @@ Prints ONE JSON line (rank 0, N = 1).  Also reports:
   cpu_baseline  the CPU oracle (oracle/, a restatement of the same
                 algorithm) on the same scenario for a bounded prefix of
                 generations, on the host's cores
+
+Other workloads (--workload), each printing one JSON line of its own:
+  batched    a parameter sweep of 4,096 single-population simulations
+             (scenarios.parameter_sweep) in one gf_batch_run, against the
+             same simulations run back to back through gf_run (a stratified
+             sample, scaled to the whole sweep); metric simulations/s
+  surrogate  trains the surrogate ensemble on batched simulations and
+             reports its predictions/s, the batched simulator's SFS/s and
+             the held-out deviance against two simulation replicates
 """
 import argparse
 import json
@@ -42,6 +51,11 @@ def parse():
     ap.add_argument("--generations", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-generations", type=int, default=0, help="generations of the CPU sample (0: auto)")
+    ap.add_argument("--workload", choices=["plain_large", "batched", "surrogate"], default="plain_large")
+    ap.add_argument("--sims", type=int, default=4096, help="batched: simulations in the sweep")
+    ap.add_argument("--plain-sample", type=int, default=32, help="batched: simulations timed back to back")
+    ap.add_argument("--train-draws", type=int, default=1000000, help="surrogate: training draws")
+    ap.add_argument("--test-draws", type=int, default=10000, help="surrogate: held-out draws")
     return ap.parse_args()
 
 
@@ -81,6 +95,109 @@ def cpu_baseline(scenarios, G_gpu, want_gens):
                        "generations, %d threads, %.1f s" % (G, G_gpu, threads, dt))
 
 
+def bench_batched(a, capi, scenarios):
+    import numpy as np
+    sims = scenarios.parameter_sweep(a.sims)
+    interval = 16
+    capi.batch_run(sims[:64], compact_interval=interval, sfs_n=50)     # warm-up
+    times = []
+    for _ in range(max(1, a.steps)):
+        t0 = time.perf_counter()
+        r = capi.batch_run(sims, compact_interval=interval, sfs_n=50)
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    # back-to-back plain runs: one simulation from each of `plain_sample`
+    # equal strata of the expected work (expected live sites x generations)
+    Nc = np.floor(2.0 * sims["N"] / (1.0 + sims["F"]))
+    work = sims["num_sites"] * 2e-9 * Nc * scenarios.harmonic(Nc - 1) * sims["num_generations"]
+    order = np.argsort(work)
+    strata = np.array_split(order, a.plain_sample)
+    rng = np.random.default_rng(7)
+    est = 0.0
+    d0 = scenarios.sweep_plain_desc(sims[strata[0][0]])
+    capi.run(d0)
+    for st in strata:
+        k = int(rng.choice(st))
+        d = scenarios.sweep_plain_desc(sims[k])
+        t0 = time.perf_counter()
+        res = capi.run(d)
+        capi.site_frequency_spectrum(res["samples"][-1]["freq"][0], 50, d.num_sites)
+        est += (time.perf_counter() - t0) * len(st)
+    return {
+        "metric": "simulations/s",
+        "value": len(sims) / dt,
+        "unit": "simulations/s",
+        "n_gpus": 1,
+        "steps": max(1, a.steps),
+        "warmup": 1,
+        "ms_per_step": 1e3 * dt,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (each simulation's own mutation-selection-equilibrium start)",
+        "config": {
+            "workload": "batched parameter sweep: %d simulations, N log-U[1e3,1e5], gamma U[-50,10], "
+                        "h in {0,0.5,1}, F U[0,0.2], mu 1e-9, 4N/10 generations, E[live] log-U[1e3,1e6], "
+                        "SFS n=50, compaction every %d generations" % (len(sims), interval),
+            "mutation_generations": int(r["live_mutation_generations"]),
+            "mutation_generations_per_s": r["live_mutation_generations"] / dt,
+            "parallelism": "single GPU",
+        },
+        "plain_back_to_back": {
+            "estimated_seconds": est,
+            "simulations_per_s": len(sims) / est,
+            "sample": "%d simulations (one per stratum of expected work), each gf_run + SFS n=50, "
+                      "scaled by stratum size" % a.plain_sample,
+            "batched_speedup": est / dt,
+        },
+    }
+
+
+def bench_surrogate(a):
+    from parallelpopgen_amd import surrogate as SG
+    t0 = time.perf_counter()
+    data = SG.make_dataset(a.train_draws + a.test_draws, seed=11)
+    n_tr = a.train_draws
+    x, y, t = data["x"][:n_tr], data["y"][:n_tr], data["t"][:n_tr]
+    t1 = time.perf_counter()
+    net, losses = SG.fit(x, y, t)
+    t_fit = time.perf_counter() - t1
+    thr = SG.threshold_from(net, data["x"][n_tr:])
+    ev = SG.evaluate(SG.Surrogate(net, thr), a.test_draws, seed=12)
+    rate = SG.prediction_rate(net)
+    sim_rate = len(data["x"]) / data["sim_seconds"]
+    return {
+        "metric": "SFS/s",
+        "value": rate,
+        "unit": "SFS predictions/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1e3 * (1 << 20) / rate,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (batched simulations of random parameter draws)",
+        "config": {
+            "workload": "surrogate: 4 MLPs 6->512x4 (GELU)->99 softmax + log-total, trained on %d draws "
+                        "(gamma U[-50,10], h {0,0.5,1}, F U[0,0.2], growth logU[1,100] over logU[10,2000] "
+                        "generations, n=100), %d held-out" % (n_tr, a.test_draws),
+            "parallelism": "single GPU",
+        },
+        "simulator_sfs_per_s": sim_rate,
+        "speedup_vs_batched_simulator": rate / sim_rate,
+        "training_seconds": t_fit,
+        "data_seconds": data["sim_seconds"],
+        "final_train_loss": float(losses[-1]),
+        "disagreement_threshold": thr,
+        "held_out": ev,
+        "deviance_ratio": ev["deviance_surrogate"] / ev["deviance_replicates"],
+        "total_seconds": time.perf_counter() - t0,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -90,6 +207,12 @@ def main():
         sys.exit(2)
     from parallelpopgen_amd import capi, scenarios
     capi.load()
+    if a.workload == "batched":
+        print(json.dumps(bench_batched(a, capi, scenarios)))
+        return
+    if a.workload == "surrogate":
+        print(json.dumps(bench_surrogate(a)))
+        return
     capi.set_kernel_timing(True)
     for _ in range(a.warmup):
         gpu_step(capi, scenarios, a.generations)

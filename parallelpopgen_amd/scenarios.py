@@ -117,3 +117,54 @@ def expected_neutral_mse(N, L, mu=1e-9, F=0.0):
     2 mu L * Nc * H(Nc-1) (sum over classes of 2 mu L / (i/Nc))."""
     Nc = int(2 * N / (1 + F))
     return 2 * mu * L * Nc * sum(1.0 / i for i in range(1, Nc))
+
+
+def harmonic(n):
+    """H(n) = sum_{i=1..n} 1/i for an integer array n >= 1 (exact sum up to
+    100, asymptotic expansion beyond: error < 1e-12)."""
+    n = np.asarray(n, np.float64)
+    small = np.cumsum(1.0 / np.arange(1, 101))
+    big = np.log(np.maximum(n, 1)) + 0.5772156649015329 + 1 / (2 * n) - 1 / (12 * n ** 2) + 1 / (120 * n ** 4)
+    return np.where(n <= 100, small[np.clip(n.astype(np.int64), 1, 100) - 1], big)
+
+
+def parameter_sweep(n=4096, seed=2024, seed_base=1000, mu=1e-9, sfs_n=50):
+    """Batched parameter sweep of single-population simulations (one row of
+    gf_batch_sim per simulation):
+      N ~ log-uniform[1e3, 1e5] diploids, gamma = 2Ns ~ U[-50, 10]
+      (s = gamma / Nc as Sim_Model::selection_constant), h in {0, 0.5, 1},
+      F ~ U[0, 0.2], mu = 1e-9, 4N/10 generations from mutation-selection
+      equilibrium, seeds (seed_base + index, 0x5eed).
+    The number of sites L is set per simulation so that the expected
+    number of segregating sites at neutral equilibrium,
+    2 mu L Nc H(Nc - 1), is log-uniform in [1e3, 1e6]."""
+    from .capi import BATCH_SIM_DTYPE
+    rng = np.random.default_rng(seed)
+    N = np.exp(rng.uniform(math.log(1e3), math.log(1e5), n)).astype(np.int32)
+    gamma = rng.uniform(-50.0, 10.0, n)
+    h = rng.choice([0.0, 0.5, 1.0], n)
+    F = rng.uniform(0.0, 0.2, n).astype(np.float32)
+    target = np.exp(rng.uniform(math.log(1e3), math.log(1e6), n))
+    Nc = np.floor((2.0 * N) / (1.0 + F.astype(np.float64))).astype(np.int64)
+    L = target / (2.0 * mu * Nc * harmonic(Nc - 1))
+    sims = np.zeros(n, BATCH_SIM_DTYPE)
+    sims["N"] = N
+    sims["F"] = F
+    sims["h"] = h
+    sims["mu"] = mu
+    sims["s"] = [selection_from_gamma(g, Ni, Fi) for g, Ni, Fi in zip(gamma, N, F)]
+    sims["num_sites"] = L.astype(np.float32)
+    sims["seed1"] = seed_base + np.arange(n)
+    sims["seed2"] = 0x5eed
+    sims["num_generations"] = (4 * N.astype(np.int64) // 10).astype(np.int32)
+    return sims
+
+
+def sweep_plain_desc(sim, compact_interval=35):
+    """the gf_run description of one row of a batched sweep"""
+    d = constant(N=int(sim["N"]), L=float(sim["num_sites"]), G=int(sim["num_generations"]), mu=float(sim["mu"]),
+                 F=float(sim["F"]), h=float(sim["h"]), s=np.float32(sim["s"]), seed1=int(sim["seed1"]),
+                 seed2=int(sim["seed2"]), compact_interval=compact_interval)
+    if sim["N1"] > 0:
+        d.params["N"][1:] = sim["N1"]
+    return d

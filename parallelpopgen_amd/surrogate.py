@@ -21,6 +21,7 @@ threshold fixed on held-out data.
 """
 import ctypes as C
 import math
+import time
 
 import numpy as np
 
@@ -210,3 +211,84 @@ class Surrogate:
             sub = {k: v[idx] for k, v in p.items()}
             counts[idx] = simulate(sub, seeds[idx])
         return counts, fallback
+
+
+# ---------------- data, training, evaluation ----------------
+
+def make_dataset(n, seed, chunk=8192):
+    """n parameter draws and their simulated SFS (one batched simulation
+    each); returns features, targets and the simulation wall time"""
+    rng = np.random.default_rng(seed)
+    p = draw(n, rng)
+    seeds = rng.integers(1, 2 ** 31 - 1, n)
+    t0 = time.perf_counter()
+    sfs = simulate(p, seeds, chunk=chunk)
+    dt = time.perf_counter() - t0
+    y, t = targets(sfs)
+    return dict(params=p, seeds=seeds, x=features(p), y=y, t=t, sfs=sfs, sim_seconds=dt)
+
+
+DEFAULT_SCHEDULE = ((1e-3, 12), (3e-4, 8), (1e-4, 5))
+
+
+def fit(x, y, t, ensemble=4, seed=1, schedule=DEFAULT_SCHEDULE, batch=512, w_tot=1.0):
+    """trains a fresh ensemble with Adam through a piecewise-constant
+    learning-rate schedule of (lr, epochs) pairs; returns (net, losses)"""
+    net = Network(ensemble=ensemble, seed=seed)
+    n = len(x) // batch * batch
+    losses = []
+    for i, (lr, ep) in enumerate(schedule):
+        losses += list(net.train(x[:n], y[:n], t[:n], epochs=ep, batch=batch, lr=lr, w_tot=w_tot, seed=seed + i))
+    return net, np.array(losses)
+
+
+def threshold_from(net, x_val, quantile=0.99):
+    """disagreement threshold: the given quantile of the ensemble
+    disagreement on held-out inputs"""
+    _, _, dis = net.predict(x_val)
+    return float(np.quantile(dis, quantile))
+
+
+def evaluate(sur, n, seed):
+    """fresh draws, each simulated twice with independent seeds (replicates
+    A and B).  Returns the mean per-bin Poisson deviance of A against B (the
+    replicate noise floor), of A against the surrogate (with its fallback),
+    the fallback fraction, and the simulator and surrogate rates in SFS/s."""
+    rng = np.random.default_rng(seed)
+    p = draw(n, rng)
+    sa = rng.integers(1, 2 ** 31 - 1, n)
+    sb = rng.integers(1, 2 ** 31 - 1, n)
+    sc = rng.integers(1, 2 ** 31 - 1, n)   # fallback simulations: a third, independent replicate
+    t0 = time.perf_counter()
+    A = simulate(p, sa)
+    t_sim = time.perf_counter() - t0
+    B = simulate(p, sb)
+    t0 = time.perf_counter()
+    pred, fb = sur.predict(p, sc)
+    t_sur = time.perf_counter() - t0
+    x = features(p)
+    sfs, tot, _ = sur.net.predict(x)
+    net_only = expected_counts(sfs, tot)
+    return dict(
+        deviance_replicates=float(poisson_deviance(A, B).mean()),
+        deviance_surrogate=float(poisson_deviance(A, pred).mean()),
+        deviance_network_only=float(poisson_deviance(A, net_only).mean()),
+        fallback_fraction=float(fb.mean()),
+        simulator_sfs_per_s=n / t_sim,
+        surrogate_with_fallback_sfs_per_s=n / t_sur,
+    )
+
+
+def prediction_rate(net, n=1 << 20, repeats=3, seed=0):
+    """ensemble predictions per second for n inputs already on the host
+    (includes the host<->device copies of inputs and outputs)"""
+    rng = np.random.default_rng(seed)
+    x = features(draw(n, rng))
+    net.predict(x[:8192])
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        net.predict(x)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return n / best
