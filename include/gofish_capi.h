@@ -225,6 +225,7 @@ int32_t gf_sg_forward(gf_sg_model *model, const float *x, int32_t n, float *logi
 #define GF_MATH_DIV         8
 #define GF_MATH_ROUND_AWAY  9
 #define GF_MATH_COPY       10   /* out = in0 (float): a plain copy kernel for counter calibration */
+#define GF_MATH_F2I        11   /* out (int32) = saturating float -> int conversion of in0 */
 int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void *in2, const void *in3,
                        void *out, int64_t n, int32_t device);
 

@@ -807,6 +807,7 @@ void orc_pois_draw_array(const uint32_t *bits, const float *mean, const float *N
     for (int64_t i = 0; i < n; i++) out[i] = pois_draw(bits[i], mean[i], mean[i], N[i]);
 }
 void orc_round_away_array(const float *in, float *out, int64_t n) { for (int64_t i = 0; i < n; i++) out[i] = round_away(in[i]); }
+void orc_f2i_array(const float *in, int32_t *out, int64_t n) { for (int64_t i = 0; i < n; i++) out[i] = f2i_sat(in[i]); }
 
 /* one generation of drift for a set of mutations (P populations) */
 int32_t orc_drift_step(int32_t P, int32_t M, float *x /*[M][P]*/, const gf_mutid *ids, int32_t g,

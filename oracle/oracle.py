@@ -40,6 +40,7 @@ def load():
         "orc_binom_draw_array": (None, [vp, vp, vp, vp, vp, i64]),
         "orc_pois_draw_array": (None, [vp, vp, vp, vp, i64]),
         "orc_round_away_array": (None, [vp, vp, i64]),
+        "orc_f2i_array": (None, [vp, vp, i64]),
         "orc_drift_step": (i32, [i32, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
         "orc_mse_counts": (i32, [vp, i32, f32, i32, i32, vp, i32]),
         "orc_new_count": (i32, [i32, f32, f32, i32, i32, i32, i32]),
@@ -102,6 +103,10 @@ def normcdfinv(x):
 
 def round_away(x):
     return _unary("orc_round_away_array", x)
+
+
+def f2i(x):
+    return _unary("orc_f2i_array", x, out_dtype=np.int32)
 
 
 def poiscdfinv(r, mean):

@@ -266,7 +266,7 @@ def set_generations_per_launch(k):
 
 
 MATH_OPS = dict(philox=0, u01=1, exp=2, log=3, normcdfinv=4, binom_draw=5, pois_draw=6, sqrt=7, div=8,
-                round_away=9, copy=10)
+                round_away=9, copy=10, f2i=11)
 
 
 def device_math(op, *inputs, device=-1):
@@ -284,7 +284,7 @@ def device_math(op, *inputs, device=-1):
         n = a0.size
         rest = [np.ascontiguousarray(np.broadcast_to(x, a0.shape), np.float32) for x in inputs[1:]]
         ins = [a0] + rest + [None] * (3 - len(rest))
-        out = np.empty(n, np.int32 if op in ("binom_draw", "pois_draw") else np.float32)
+        out = np.empty(n, np.int32 if op in ("binom_draw", "pois_draw", "f2i") else np.float32)
     ptrs = [None if x is None else x.ctypes.data_as(C.c_void_p) for x in ins]
     check(lib.gf_device_math(code, ptrs[0], ptrs[1], ptrs[2], ptrs[3], out.ctypes.data_as(C.c_void_p), n, device))
     return out

@@ -35,6 +35,7 @@ static __global__ void k_math(int op, const void *in0, const void *in1, const vo
         case GF_MATH_DIV: fo[i] = f0[i] / f1[i]; break;
         case GF_MATH_ROUND_AWAY: fo[i] = round_away(f0[i]); break;
         case GF_MATH_COPY: fo[i] = f0[i]; break;
+        case GF_MATH_F2I: io[i] = f2i_sat(f0[i]); break;
         default: break;
         }
     }
@@ -45,7 +46,7 @@ static __global__ void k_math(int op, const void *in0, const void *in1, const vo
 extern "C" int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void *in2, const void *in3,
                                   void *out, int64_t n, int32_t device) {
     using namespace gf;
-    if (n <= 0 || !in0 || !out || op < 0 || op > GF_MATH_COPY) { set_error("gf_device_math: bad arguments"); return GF_ERR_ARG; }
+    if (n <= 0 || !in0 || !out || op < 0 || op > GF_MATH_F2I) { set_error("gf_device_math: bad arguments"); return GF_ERR_ARG; }
     GF_TRY(select_device(device));
     const size_t in_bytes = op == GF_MATH_PHILOX ? 24 : 4;
     const size_t out_bytes = op == GF_MATH_PHILOX ? 16 : 4;

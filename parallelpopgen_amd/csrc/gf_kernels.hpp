@@ -61,6 +61,26 @@ struct SegLaunch {
     int   pad;
 };
 
+/* Model rows and segment descriptors are read-only for a whole launch.
+ * Reading them through the constant address space lets the compiler use
+ * scalar loads (and uniform branches) where the index is wave-uniform,
+ * which it cannot prove for a generic pointer next to the kernel's own
+ * stores. */
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *as_const(const T *p) {
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
+/* a segment descriptor read with scalar loads (see as_const) */
+template <int P>
+__device__ __forceinline__ SegLaunch<P> load_seg(const SegLaunch<P> *p) {
+    const auto &c = *as_const(p);
+    SegLaunch<P> S;
+    S.rows = c.rows; S.live = c.live; S.base = c.base; S.M0 = c.M0; S.Mend = c.Mend; S.cum_base = c.cum_base;
+    S.K = c.K; S.g0 = c.g0; S.t0 = c.t0; S.k0 = c.k0; S.k1 = c.k1; S.pad = c.pad;
+    return S;
+}
+
 /* keep rule of the compaction flag (reference go_fish_impl.cuh:153-166) */
 template <int P>
 __device__ __forceinline__ bool keep_rule(const float *x, int present_mask, bool preserved) {
@@ -87,8 +107,8 @@ __device__ __forceinline__ bool keep_rule(const float *x, int present_mask, bool
  * is drifted in this launch), also for a population that is empty in this
  * generation, so that it is current if the population is founded later in
  * the same block. */
-template <int P>
-__device__ __forceinline__ void generation_update(float *x, const GenRow<P> &R, int g, bool refresh, u32x4 *rc,
+template <int P, typename Row>
+__device__ __forceinline__ void generation_update(float *x, const Row &R, int g, bool refresh, u32x4 *rc,
                                                   uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop) {
     float nx[P];
 #pragma unroll
@@ -151,7 +171,8 @@ __global__ __launch_bounds__(WG) void k_advance(float *__restrict__ freq, int64_
     uint64_t cnt = 0;
     for (int tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
         const int s = BATCH ? tile_seg[tile] : 0;
-        const SegLaunch<P> &S = BATCH ? segs[s] : seg0;
+        SegLaunch<P> S = seg0;
+        if (BATCH) S = load_seg<P>(segs + s);
         const int li = (BATCH ? (tile - tile_first[s]) : tile) * WG + (int)threadIdx.x;
         const bool valid = li < S.Mend;
         const int64_t slot = S.base + li;
@@ -190,8 +211,9 @@ __global__ __launch_bounds__(WG) void k_advance(float *__restrict__ freq, int64_
             }
         }
         u32x4 rc[P];
+        const auto *crows = as_const(rows);
         for (int t = 0; t < S.K; t++) {
-            const GenRow<P> &R = rows[t];
+            const auto &R = crows[t];
             const int g = S.g0 + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
@@ -326,8 +348,9 @@ __global__ __launch_bounds__(WG) void k_advance_fused(const FusedLaunch F, const
         }
         u32x4 rc[P];
         bool keep = false;
+        const auto *crows = as_const(rows);
         for (int t = 0; t < S.K; t++) {
-            const GenRow<P> &R = rows[t];
+            const auto &R = crows[t];
             const int g = S.g0 + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);

@@ -114,6 +114,25 @@ GF_HD float log_f(float x) {
     return fe * 0.693359375f + (lm + fe * -2.12194440e-4f);
 }
 
+/* log_f without its special cases: the same operations, so the same value,
+ * for finite normal x > 0 */
+GF_HD float log_f_mid(float x) {
+    const uint32_t b = f2bits(x);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    float q = 0.111111111f;
+    q = q * s2 + 0.142857143f;
+    q = q * s2 + 0.2f;
+    q = q * s2 + 0.333333333f;
+    float lm = (s + s) + (s + s) * (s2 * q);
+    float fe = (float)e;
+    return fe * 0.693359375f + (lm + fe * -2.12194440e-4f);
+}
+
 GF_HD float pow_f(float a, float b) {
     if (a == 1.0f || b == 0.0f) return 1.0f;
     if (a == 0.0f) return b > 0.0f ? 0.0f : __builtin_inff();
@@ -136,7 +155,7 @@ GF_HD float normcdfinv_f(float p) {
         b3 = 6.7187563600e+01f; b2 = 7.8757757664e+01f; b1 = 1.7895169469e+01f;
     } else {
         float t = q < 0.0f ? p : 1.0f - p;
-        t = __builtin_sqrtf(-log_f(t));
+        t = __builtin_sqrtf(-log_f_mid(t));   /* 2^-33 <= t < 0.075: normal, positive, finite */
         b3 = 0.0f;
         if (t <= 5.0f) {
             r = t - 1.6f;
@@ -162,11 +181,20 @@ GF_HD float round_away(float v) {
     return t;
 }
 
+/* float -> int32 with saturation (NaN -> 0, >= 2^31 -> INT_MAX,
+ * <= -2^31 -> INT_MIN), truncating toward zero otherwise.  On the device
+ * this is exactly v_cvt_i32_f32's defined behaviour. */
 GF_HD int f2i_sat(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+#else
     if (v != v) return 0;
     if (v >= 2147483648.0f) return 2147483647;
     if (v <= -2147483648.0f) return (-2147483647 - 1);
     return (int)v;
+#endif
 }
 
 /* early-exit means of the truncated CDF inversions: after term j the
@@ -188,7 +216,7 @@ GF_HD int exit_mean(int j) {
 
 /* correctly rounded reciprocals fl(1/j), j = 1..69, of the Poisson CDF
  * recurrence emu_j = emu_(j-1) * (mean * (1/j)) */
-static constexpr float kRecip[69] = {
+alignas(16) static constexpr float kRecip[72] = {
     0x1.0000000000000p+0f, 0x1.0000000000000p-1f, 0x1.5555560000000p-2f, 0x1.0000000000000p-2f,
     0x1.99999a0000000p-3f, 0x1.5555560000000p-3f, 0x1.24924a0000000p-3f, 0x1.0000000000000p-3f,
     0x1.c71c720000000p-4f, 0x1.99999a0000000p-4f, 0x1.745d180000000p-4f, 0x1.5555560000000p-4f,
@@ -207,6 +235,7 @@ static constexpr float kRecip[69] = {
     0x1.0c97140000000p-6f, 0x1.0842100000000p-6f, 0x1.0410420000000p-6f, 0x1.0000000000000p-6f,
     0x1.f81f820000000p-7f, 0x1.f07c200000000p-7f, 0x1.e9131a0000000p-7f, 0x1.e1e1e20000000p-7f,
     0x1.dae6080000000p-7f,
+    0.0f, 0.0f, 0.0f,   /* padding: the inversion reads four entries at a time */
 };
 
 /* First term index at which the truncated inversion stops whatever the
@@ -226,38 +255,51 @@ GF_HD int exit_index(float mean) {
     return kExitIndex[k];
 }
 
-/* true if `b` holds for any active lane of the calling wave (device), or
- * for the single calling thread (host) */
-GF_HD bool wave_any(bool b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __ballot(b) != 0;
-#else
-    return b;
-#endif
+/* exp_f without its range checks: the same operations, so the same value,
+ * for -87 <= x <= 88 (no NaN, no overflow, the scale 2^n stays normal) */
+GF_HD float exp_f_mid(float x) {
+    float t = x * 1.44269502162933349609375f;
+    float n = __builtin_floorf(t + 0.5f);
+    float r = x - n * 0.693359375f;
+    r = r - n * -2.12194440e-4f;
+    float p = 1.98412698e-4f;
+    p = p * r + 1.38888889e-3f;
+    p = p * r + 8.33333333e-3f;
+    p = p * r + 4.16666667e-2f;
+    p = p * r + 1.66666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return p * pow2i((int)n);
+}
+
+/* one term of the Poisson CDF recurrence; true if the inversion stops at it */
+GF_HD bool pois_term(float &emu, float &cdf, float mean, float rj, float r, int j, int jstop) {
+    emu = emu * (mean * rj);
+    cdf = cdf + emu;
+    return cdf >= r || j == jstop;
 }
 
 /* Poisson CDF inversion truncated as in the reference (shared.cuh:168-249):
  * the first j with CDF(j) >= r, or the early-exit index of `mean` if that
- * comes first, or 70.  On the device the loop runs in step across the wave
- * while any lane is still searching, and a lane that has found its j keeps
- * it (the terms it computes after that are ignored); this returns the same
- * j as a per-lane early return without one nested branch per term. */
+ * comes first, or 70.  Callers pass 0 <= mean <= 6 (the Poisson tails of
+ * the approximate binomial), where exp_f_mid(-mean) == exp_f(-mean).  The
+ * loop stays rolled (lanes leave it as they finish, with no nested branch
+ * per term) and takes four reciprocals per trip in one scalar load. */
 GF_HD int poiscdfinv(float r, float mean) {
-    float emu = exp_f(-1.0f * mean);
+    float emu = exp_f_mid(-1.0f * mean);
     float cdf = emu;
+    if (cdf >= r) return 0;
     const int jstop = exit_index(mean);
-    bool done = cdf >= r;
-    int k = 0;
-#pragma unroll 2
-    for (int jj = 1; jj < 70; jj++) {
-        if (!wave_any(!done)) break;
-        emu = emu * (mean * kRecip[jj - 1]);
-        cdf = cdf + emu;
-        const bool hit = !done && ((cdf >= r) || (jj == jstop));
-        k = hit ? jj : k;
-        done = done || hit;
+#pragma nounroll
+    for (int j = 1;; j += 4) {
+        const float c0 = kRecip[j - 1], c1 = kRecip[j], c2 = kRecip[j + 1], c3 = kRecip[j + 2];
+        if (pois_term(emu, cdf, mean, c0, r, j, jstop)) return j;
+        if (j == 69) return 70;
+        if (pois_term(emu, cdf, mean, c1, r, j + 1, jstop)) return j + 1;
+        if (pois_term(emu, cdf, mean, c2, r, j + 2, jstop)) return j + 2;
+        if (pois_term(emu, cdf, mean, c3, r, j + 3, jstop)) return j + 3;
     }
-    return done ? k : 70;
 }
 
 /* binomial CDF inversion (N < 100 only): kept out of line, it is off the

@@ -43,6 +43,19 @@ def test_unary_bit_exact(gpu, oracle, op):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), op
 
 
+def test_saturating_float_to_int(gpu, oracle):
+    rng = np.random.default_rng(15)
+    edges = np.array([np.nan, -np.nan, np.inf, -np.inf, 2.0 ** 31, -2.0 ** 31, 2.0 ** 31 - 128, -2.0 ** 31 + 128,
+                      3e9, -3e9, 1e38, -1e38, 0.0, -0.0, 0.5, -0.5, 0.999999, -0.999999, 1.5, -1.5, 1e-45, 16777217.0],
+                     np.float32)
+    x = np.concatenate([edges, rng.uniform(-3e9, 3e9, 1 << 18).astype(np.float32),
+                        rng.uniform(-1e3, 1e3, 1 << 18).astype(np.float32)])
+    got = gpu.device_math("f2i", x)
+    assert np.array_equal(got, oracle.f2i(x))
+    assert got[0] == 0 and got[2] == 2147483647 and got[3] == -2147483648 and got[4] == 2147483647
+    assert got[5] == -2147483648 and got[14] == 0 and got[17] == 0 and got[19] == -1
+
+
 def test_u01_and_div_bit_exact(gpu, oracle):
     rng = np.random.default_rng(13)
     u = rng.integers(0, 2**32, 1 << 20, dtype=np.uint64).astype(np.uint32)

@@ -193,3 +193,14 @@ def test_pois_draw_moments(oracle):
         k = oracle.pois_draw(bits, np.float32(lam), np.float32(1e12)).astype(np.float64)
         assert abs(k.mean() - lam) < 5 * np.sqrt(lam / len(k)) + 1e-3
         assert abs(k.var() / lam - 1) < 0.03
+
+
+def test_f2i_saturates_and_truncates(oracle):
+    x = np.array([np.nan, np.inf, -np.inf, 2.0 ** 31, -2.0 ** 31, 3e9, -3e9, 2.5, -2.5, 0.99, -0.99, 123456.7],
+                 np.float32)
+    want = np.array([0, 2 ** 31 - 1, -2 ** 31, 2 ** 31 - 1, -2 ** 31, 2 ** 31 - 1, -2 ** 31, 2, -2, 0, 0, 123456],
+                    np.int64)
+    assert np.array_equal(oracle.f2i(x).astype(np.int64), want)
+    rng = np.random.default_rng(3)
+    y = rng.uniform(-2e9, 2e9, 100000).astype(np.float32)
+    assert np.array_equal(oracle.f2i(y).astype(np.int64), np.trunc(y.astype(np.float64)).astype(np.int64))
