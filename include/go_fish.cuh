@@ -15,9 +15,14 @@
  * form s(x) = slope*x + intercept, and hands the tables to the gfx950
  * library through the C ABI (gf_run, include/gofish_capi.h); the results
  * are copied into allele_trajectories.  Every Sim_Model selection functor has
- * that linear form; a user-written selection functor provides it through a
+ * that linear form; a user-written selection functor may provide it through a
  *   void linear_form(int population, int generation, float &slope, float &intercept) const
- * member (its operator() keeps the reference signature).
+ * member.  Any other selection functor -- the reference's contract,
+ *   __device__ float operator()(int population, int generation, float freq) const
+ * -- is compiled into the library's drift kernel in the calling program
+ * (parallelpopgen_amd/csrc/gf_drift.hpp) and passed to the library through
+ * gf_run_with_selection, as the reference compiles migration_selection_drift
+ * with the user's functor (go_fish_impl.cuh:93-124).
  */
 #ifndef GO_FISH_API_H_
 #define GO_FISH_API_H_
@@ -31,6 +36,7 @@
 
 #include "go_fish_data_struct.h"
 #include "gofish_capi.h"
+#include "../parallelpopgen_amd/csrc/gf_drift.hpp"
 
 namespace Sim_Model {
 
@@ -350,6 +356,9 @@ struct bool_piecewise {
 /* ================= lowering of selection functors ================= */
 namespace go_fish_details {
 
+/* true if a selection functor has a per-(population, generation) linear
+ * form: a linear_form member, and for the composite Sim_Model types, parts
+ * that all have one */
 template <typename T>
 struct has_linear_form {
     template <typename U>
@@ -358,15 +367,58 @@ struct has_linear_form {
     static std::false_type test(...);
     static constexpr bool value = decltype(test<T>(0))::value;
 };
+template <typename A, typename B>
+struct has_linear_form<Sim_Model::selection_population_specific<A, B>>
+    : std::integral_constant<bool, has_linear_form<A>::value && has_linear_form<B>::value> {};
+template <typename A, typename B>
+struct has_linear_form<Sim_Model::selection_piecewise<A, B>>
+    : std::integral_constant<bool, has_linear_form<A>::value && has_linear_form<B>::value> {};
 
 template <typename Sel>
 inline void lower_selection(const Sel &sel, int population, int generation, float &slope, float &intercept) {
-    static_assert(has_linear_form<Sel>::value,
-                  "GO_Fish::run_sim: the selection functor must be a Sim_Model selection type or provide "
-                  "`void linear_form(int population, int generation, float &slope, float &intercept) const` "
-                  "with s(freq) = slope*freq + intercept");
-    sel.linear_form(population, generation, slope, intercept);
+    if constexpr (has_linear_form<Sel>::value) {
+        sel.linear_form(population, generation, slope, intercept);
+    } else {
+        slope = 0.0f;
+        intercept = 0.0f;   /* not used: selection comes from the functor itself */
+    }
 }
+
+/* Launchers handed to gf_run_with_selection for a functor without a linear
+ * form: the library's drift kernel and equilibrium selection table,
+ * instantiated here with the functor (user = the functor object). */
+template <typename Sel>
+struct selection_hooks {
+    static int32_t mse_table(void *user, int32_t pop, int32_t gen, int32_t Nc, float *table, void *stream) {
+        const Sel &f = *static_cast<const Sel *>(user);
+        const int grid = (Nc + 255) / 256 < 1024 ? (Nc + 255) / 256 : 1024;
+        hipLaunchKernelGGL((gf::k_mse_sel_table<Sel>), dim3(grid > 0 ? grid : 1), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), f, pop, gen, Nc, table);
+        return hipGetLastError() == hipSuccess ? GF_OK : GF_ERR_DEVICE;
+    }
+    template <int P>
+    static void launch(const Sel &f, const void *fused, const void *segment, int grid, hipStream_t st) {
+        hipLaunchKernelGGL((gf::k_advance_fused<P, gf::FunctorSel<Sel>>), dim3(grid), dim3(gf::WG), 0, st,
+                           *static_cast<const gf::FusedLaunch *>(fused),
+                           *static_cast<const gf::SegLaunch<P> *>(segment), gf::FunctorSel<Sel>{f});
+    }
+    static int32_t advance(void *user, int32_t P, const void *fused, const void *segment, int32_t grid, void *stream) {
+        const Sel &f = *static_cast<const Sel *>(user);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        switch (P) {
+        case 1: launch<1>(f, fused, segment, grid, st); break;
+        case 2: launch<2>(f, fused, segment, grid, st); break;
+        case 3: launch<3>(f, fused, segment, grid, st); break;
+        case 4: launch<4>(f, fused, segment, grid, st); break;
+        case 5: launch<5>(f, fused, segment, grid, st); break;
+        case 6: launch<6>(f, fused, segment, grid, st); break;
+        case 7: launch<7>(f, fused, segment, grid, st); break;
+        case 8: launch<8>(f, fused, segment, grid, st); break;
+        default: return GF_ERR_ARG;
+        }
+        return hipGetLastError() == hipSuccess ? GF_OK : GF_ERR_DEVICE;
+    }
+};
 
 } /* namespace go_fish_details */
 
@@ -475,7 +527,17 @@ inline void run_sim(allele_trajectories &all_results, const Functor_mutation mu_
     d.sample = samp.data();
     d.device = dev;
     gf_result *res = NULL;
-    if (gf_run(&d, &res) != GF_OK) {
+    int32_t rc;
+    if constexpr (go_fish_details::has_linear_form<Functor_selection>::value) {
+        rc = gf_run(&d, &res);
+    } else {
+        gf_selection_hooks hooks;
+        hooks.user = const_cast<void *>(static_cast<const void *>(&sel_coeff));
+        hooks.mse_table = &go_fish_details::selection_hooks<Functor_selection>::mse_table;
+        hooks.advance = &go_fish_details::selection_hooks<Functor_selection>::advance;
+        rc = gf_run_with_selection(&d, &hooks, &res);
+    }
+    if (rc != GF_OK) {
         std::fprintf(stderr, "%s\n", gf_last_error());
         std::exit(1);
     }

@@ -108,6 +108,36 @@ typedef struct gf_timing {
  * :137-176, scatter_arrays go_fish_impl.cu:147-189). */
 int32_t gf_run(const gf_sim_desc *desc, gf_result **out);
 
+/* Selection evaluated by code compiled in the caller's program.  The
+ * reference takes any __device__ float(int population, int generation,
+ * float freq) selection functor and compiles migration_selection_drift
+ * with it in the user's translation unit (go_fish_impl.cuh:93-124, called
+ * from run_sim :645-714; the functor contract is
+ * template_inline_simulation_functors.cuh:35-37).  gf_run lowers selection
+ * to a per-(population, generation) linear form; for a functor without
+ * one, include/go_fish.cuh instantiates the library's drift kernel with the
+ * functor in the caller's program and passes these launchers:
+ *   mse_table : fill `table` (device memory, 3*Nc floats) with the
+ *               functor at the equilibrium-start points of population
+ *               `population` (k_mse_sel_table layout, gf_drift.hpp), on
+ *               `stream` (a hipStream_t);
+ *   advance   : launch k_advance_fused<num_populations, functor> with the
+ *               launch descriptors `fused` (gf::FusedLaunch) and `segment`
+ *               (gf::SegLaunch<num_populations>) on `grid` workgroups of
+ *               256 threads, on `stream`.
+ * Each returns GF_OK or a GF_ERR_* code. */
+typedef struct gf_selection_hooks {
+    void *user;
+    int32_t (*mse_table)(void *user, int32_t population, int32_t generation, int32_t Nc, float *table,
+                         void *stream);
+    int32_t (*advance)(void *user, int32_t num_populations, const void *fused, const void *segment,
+                       int32_t grid, void *stream);
+} gf_selection_hooks;
+
+/* gf_run with selection from `hooks` instead of the sel_slope /
+ * sel_intercept columns of desc->params (which are then ignored). */
+int32_t gf_run_with_selection(const gf_sim_desc *desc, const gf_selection_hooks *hooks, gf_result **out);
+
 /* Result accessors (reference allele_trajectories accessors,
  * inline_go_fish_data_struct.hpp:263-323). */
 int32_t gf_result_num_samples(const gf_result *r);

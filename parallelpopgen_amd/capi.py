@@ -85,6 +85,7 @@ SIGNATURES = {
     "gf_batch_live_mutation_generations": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     "gf_batch_timing": (C.c_int32, [C.c_void_p, C.POINTER(gf_timing)]),
     "gf_batch_free": (None, [C.c_void_p]),
+    "gf_run_with_selection": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gf_sg_create": (C.c_int32, [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     "gf_sg_free": (None, [C.c_void_p]),
     "gf_sg_num_params": (C.c_int32, [C.c_void_p]),

@@ -259,6 +259,8 @@ struct BatchRun {
         const int kb = std::max(1, std::min({KMAX, interval > 0 ? interval : KMAX, generations_per_launch()}));
         DevBuf<unsigned long long> d_live;
         GF_TRY(d_live.alloc(1));
+        DevBuf<unsigned int> d_ticket;
+        GF_TRY(d_ticket.alloc(1));
         GF_HIP(hipMemsetAsync(d_live.p, 0, sizeof(unsigned long long), st));
 
         /* device scratch */
@@ -414,11 +416,12 @@ struct BatchRun {
             GF_HIP(hipMemcpyAsync(d_tile_seg.p, tmeta.data(), tmeta.size() * sizeof(int), hipMemcpyHostToDevice, st));
             if (ntl > 0) {
                 const int grid = std::min(ntl, num_cus() * 8);
+                GF_HIP(hipMemsetAsync(d_ticket.p, 0, sizeof(unsigned int), st));
                 hipEvent_t ae = nullptr;
                 t_adv.begin(st, ae);
                 hipLaunchKernelGGL((k_advance<1, true>), dim3(grid), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
                                    h_sl.p[0], (const SegLaunch<1> *)d_sl.p, (const int *)d_tile_seg.p,
-                                   (const int *)(d_tile_seg.p + ntl), ntl);
+                                   (const int *)(d_tile_seg.p + ntl), ntl, d_ticket.p);
                 GF_HIP(hipGetLastError());
                 t_adv.end(ae);
             }

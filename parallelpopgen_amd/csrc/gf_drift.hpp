@@ -1,0 +1,503 @@
+/*
+ * gf_drift.hpp -- the GO_Fish drift kernel (gfx950) and the types it shares
+ * with the host code.
+ *
+ * Layout in HBM (one "segment" = one simulation; the plain run_sim path is a
+ * batch of one segment):
+ *   freq  : float [P][cap]  structure of arrays, row p = population p
+ *   ids   : gf_mutid [cap]  16 B per slot (origin gen, origin pop, threadID, 0)
+ * A thread owns one mutation slot for all P populations, so migration is a
+ * register-local dot product and the update is done in place.
+ *
+ * The advance kernels run up to KMAX generations per launch ("temporal
+ * blocking"): each slot is read once, advanced through K generations of
+ * migration, selection and drift in registers, and written once.  Mutations
+ * born inside the block are materialised in their slots at the generation
+ * they arise and drifted from the next generation on, exactly as the
+ * generation-by-generation order of the reference (go_fish_impl.cuh:645-714)
+ * prescribes.
+ *
+ * This header is also compiled into programs that pass their own selection
+ * functor to GO_Fish::run_sim (include/go_fish.cuh): k_advance_fused and
+ * k_mse_sel_table are then instantiated there with that functor, and every
+ * floating-point function here keeps FMA contraction off itself, so the
+ * results do not depend on the caller's -ffp-contract setting.
+ */
+#ifndef GF_DRIFT_HPP_
+#define GF_DRIFT_HPP_
+
+#include "gf_math.hpp"
+#include "../../include/gofish_capi.h"
+
+namespace gf {
+
+constexpr int KMAX = 64;   /* generations per advance launch (lane-indexed counters) */
+constexpr int WG = 256;    /* threads per workgroup = slots per tile */
+
+/* Per-generation model row of one segment (host-built, read through scalar
+ * loads: every field is uniform across a tile). */
+template <int P>
+struct alignas(16) GenRow {
+    double invN[P];       /* 1/(double)fN.  For floats a and b, (float)((double)a * (1/(double)b))
+                           * equals the correctly rounded a/b: a/b has a denominator below 2^24, so
+                           * it is never a float rounding midpoint and lies at least 2^-49 (relative)
+                           * from one, while the double product is within 2^-52 of a/b. */
+    int   Nc[P];          /* effective chromosomes; <= 0: population not drifted -> 0 */
+    float fN[P];          /* (float)Nc */
+    float sa[P], sb[P];   /* selection s(x) = sa*x + sb */
+    float c1[P], c2[P];   /* (F+h)-(h*F), (F+2h)-(2h*F) */
+    float mig[P][P];      /* [from][to] */
+    float newfreq[P];     /* 1/Nc of new mutations */
+    int   newcum[P + 1];  /* new mutations since run start before (g,p); [P] = after g */
+    int   present_mask;   /* bit p: demography(p,g) > 0 */
+    int   neutral_mask;   /* bit p: selection identically 0 at g */
+};
+
+/* Per-launch description of one segment. */
+template <int P>
+struct SegLaunch {
+    const GenRow<P> *rows;        /* rows[t] = generation g0+1+t */
+    unsigned long long *live;     /* plain: [run gens][64]; batch: single counter */
+    int64_t base;                 /* first slot of the segment */
+    int   M0;                     /* slots in use before the block */
+    int   Mend;                   /* slots in use after the block */
+    int   cum_base;               /* rows[0].newcum[0] */
+    int   K;                      /* generations of this segment in the block */
+    int   g0;                     /* generation before the block */
+    int   t0;                     /* run index (g0+1 - start) of rows[0] */
+    uint32_t k0, k1;              /* Philox key */
+    int   pad;
+};
+
+/* Model rows and segment descriptors are read-only for a whole launch.
+ * Reading them through the constant address space lets the compiler use
+ * scalar loads (and uniform branches) where the index is wave-uniform,
+ * which it cannot prove for a generic pointer next to the kernel's own
+ * stores. */
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *as_const(const T *p) {
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
+/* a segment descriptor read with scalar loads (see as_const) */
+template <int P>
+__device__ __forceinline__ SegLaunch<P> load_seg(const SegLaunch<P> *p) {
+    const auto &c = *as_const(p);
+    SegLaunch<P> S;
+    S.rows = c.rows; S.live = c.live; S.base = c.base; S.M0 = c.M0; S.Mend = c.Mend; S.cum_base = c.cum_base;
+    S.K = c.K; S.g0 = c.g0; S.t0 = c.t0; S.k0 = c.k0; S.k1 = c.k1; S.pad = c.pad;
+    return S;
+}
+
+/* keep rule of the compaction flag (reference go_fish_impl.cuh:153-166) */
+template <int P>
+__device__ __forceinline__ bool keep_rule(const float *x, int present_mask, bool preserved) {
+    bool zero = true, one = true;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        if (present_mask & (1 << p)) {
+            zero = zero && (x[p] <= 0.0f);
+            one = one && (x[p] >= 1.0f);
+        }
+    }
+    return !(zero || one) || preserved;
+}
+
+/* Selection evaluators: s(population, generation, frequency after
+ * migration) for the row R of that generation.  LinearSel is the library's
+ * own, the per-(population, generation) linear form the host lowers every
+ * Sim_Model selection functor to; FunctorSel wraps a caller's functor with
+ * the reference signature __device__ float(int population, int generation,
+ * float freq) (template_inline_simulation_functors.cuh:35-37). */
+struct LinearSel {
+    template <typename Row>
+    __device__ __forceinline__ float operator()(int p, int, float x, const Row &R) const {
+        GF_STRICT_FP
+        return R.sa[p] * x + R.sb[p];
+    }
+};
+
+template <typename F>
+struct FunctorSel {
+    F f;
+    template <typename Row>
+    __device__ __forceinline__ float operator()(int p, int g, float x, const Row &) const { return f(p, g, x); }
+};
+
+/* ---------------- migration, selection and drift of one generation ----------------
+ * Reference migration_selection_drift (go_fish_impl.cuh:93-124): each
+ * population's frequency after migration (weighted sum over the source
+ * populations), selection (Eq. 13 of the reference's model, with F and h)
+ * and an approximate binomial(N, x') drift draw, clamped to [0, N].
+ *
+ * rc[p] caches the lane's Philox block for generations 4b..4b+3 (the
+ * reference's float4 packing, go_fish_impl.cuh:107).  It is fetched when
+ * `refresh` is set (at a block boundary, or in the first generation a slot
+ * is drifted in this launch), also for a population that is empty in this
+ * generation, so that it is current if the population is founded later in
+ * the same block. */
+template <int P, typename Row, typename Sel>
+__device__ __forceinline__ void generation_update(float *x, const Row &R, int g, bool refresh, u32x4 *rc,
+                                                  uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop,
+                                                  const Sel &sel) {
+    GF_STRICT_FP
+    float nx[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int Nc = R.Nc[p];
+        if (refresh) rc[p] = drift_block(k0, k1, tid, ogen, opop, p, g);
+        if (Nc <= 0) { nx[p] = 0.0f; continue; }
+        float im = 0.0f;
+#pragma unroll
+        for (int k = 0; k < P; k++) im = im + R.mig[k][p] * x[k];
+        float xs;
+        if (R.neutral_mask & (1 << p)) {
+            xs = im;   /* s == 0: Eq. 13 reduces exactly to x */
+        } else {
+            float s = sel(p, g, im, R);
+            s = s > -1.0f ? s : -1.0f;
+            const float om = 1.0f - im;
+            const float num = (((s * im) * im) + im) + (((R.c1[p] * s) * im) * om);
+            const float den = (((im * im) * s) + (((R.c2[p] * s) * im) * om)) + 1.0f;
+            xs = num / den;
+        }
+        const float N = R.fN[p];
+        const float mean = xs * N;
+        const float var = (1.0f - xs) * mean;
+        int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N);
+        j = j < 0 ? 0 : (j > Nc ? Nc : j);
+        nx[p] = (float)((double)(float)j * R.invN[p]);   /* == (float)j / N */
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) x[p] = nx[p];
+}
+
+/* locate a slot born inside the block: generation index t and population */
+template <int P>
+__device__ __forceinline__ void find_birth(const GenRow<P> *rows, int K, int rel, int &t, int &p) {
+    int lo = 0, hi = K - 1;          /* last t with rows[t].newcum[0] <= rel */
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rows[mid].newcum[0] <= rel) lo = mid; else hi = mid - 1;
+    }
+    t = lo;
+    p = 0;
+#pragma unroll
+    for (int q = 1; q < P; q++) if (rows[lo].newcum[q] <= rel) p = q;
+}
+
+/* Both drift kernels ask for 8 waves per SIMD (amdgpu_waves_per_eu): the
+ * compiler then keeps them within 96 SGPRs (spilling a few to VGPR lanes),
+ * which measured 5 % faster than 7 waves with no spills.
+ *
+ * Advance every slot of every segment by its K generations.
+ * BATCH=false: one segment, per-generation live counts (lane t of each wave
+ * accumulates generation t, KMAX <= 64).  BATCH=true: tile_seg maps tiles to
+ * segments and only the total of live mutation-generations is counted. */
+template <int P, bool BATCH>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_advance(float *__restrict__ freq, int64_t pitch,
+                                                gf_mutid *__restrict__ ids,
+                                                const SegLaunch<P> seg0,
+                                                const SegLaunch<P> *__restrict__ segs,
+                                                const int *__restrict__ tile_seg,
+                                                const int *__restrict__ tile_first,
+                                                int num_tiles, unsigned int *__restrict__ ticket) {
+    const int lane = threadIdx.x & 63;
+    __shared__ int sh_tile;
+    uint64_t cnt = 0;
+    /* tiles are handed out dynamically (an atomic ticket): their cost varies
+     * with the simulation they belong to */
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_tile = (int)atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int tile = sh_tile;
+        if (tile >= num_tiles) break;
+        const int s = BATCH ? tile_seg[tile] : 0;
+        SegLaunch<P> S = seg0;
+        if (BATCH) S = load_seg<P>(segs + s);
+        const int li = (BATCH ? (tile - tile_first[s]) : tile) * WG + (int)threadIdx.x;
+        const bool valid = li < S.Mend;
+        const int64_t slot = S.base + li;
+        const GenRow<P> *rows = S.rows;
+        float x[P];
+        int ogen = 0, opop = 0, born = -1;
+        uint32_t tid = 0;
+        bool preserved = false;
+#pragma unroll
+        for (int p = 0; p < P; p++) x[p] = 0.0f;
+        if (valid) {
+            if (li < S.M0) {
+#pragma unroll
+                for (int p = 0; p < P; p++) x[p] = freq[p * pitch + slot];
+                const gf_mutid id = ids[slot];
+                ogen = id.origin_generation;
+                opop = id.origin_population;
+                preserved = id.origin_threadID < 0;
+                tid = (uint32_t)(preserved ? -id.origin_threadID : id.origin_threadID);
+            } else {
+                const int rel = li - S.M0 + S.cum_base;
+                int tb, pb;
+                find_birth<P>(rows, S.K, rel, tb, pb);
+                born = tb;
+                ogen = S.g0 + 1 + tb;
+                opop = pb;
+                tid = (uint32_t)(rel - rows[tb].newcum[pb] + 1);
+#pragma unroll
+                for (int p = 0; p < P; p++) x[p] = (p == pb) ? rows[tb].newfreq[p] : 0.0f;
+                gf_mutid id;
+                id.origin_generation = ogen;
+                id.origin_population = opop;
+                id.origin_threadID = (int32_t)tid;
+                id.reserved = 0;
+                ids[slot] = id;
+            }
+        }
+        u32x4 rc[P];
+        const auto *crows = as_const(rows);
+        for (int t = 0; t < S.K; t++) {
+            const auto &R = crows[t];
+            const int g = S.g0 + 1 + t;
+            if (valid && born < t) {
+                const bool refresh = ((g & 3) == 0) || (t == born + 1);
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel());
+            }
+            const bool keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
+            const uint64_t b = __ballot(keep);
+            if (BATCH) {
+                if (lane == 0) cnt += (uint64_t)__popcll(b);
+            } else {
+                if (lane == t) cnt += (uint64_t)__popcll(b);
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int p = 0; p < P; p++) freq[p * pitch + slot] = x[p];
+        }
+        if (BATCH) {
+            /* flush per tile: segments differ between tiles */
+            if (lane == 0 && cnt) atomicAdd(S.live, (unsigned long long)cnt);
+            cnt = 0;
+        }
+    }
+    if (!BATCH) {
+        __shared__ unsigned long long sh[64];
+        if (threadIdx.x < 64) sh[threadIdx.x] = 0;
+        __syncthreads();
+        if (cnt) atomicAdd(&sh[lane], (unsigned long long)cnt);
+        __syncthreads();
+        const SegLaunch<P> &S = seg0;
+        if ((int)threadIdx.x < S.K && sh[threadIdx.x])
+            atomicAdd(&S.live[(size_t)(S.t0 + threadIdx.x) * 64 + (blockIdx.x & 63)], sh[threadIdx.x]);
+    }
+}
+
+/* ---------------- fused advance + stable compaction ----------------
+ * One launch advances every slot of a segment by K generations and, when
+ * `compact` is set, writes only the survivors (compaction keep rule at the
+ * last generation of the block) in their original order into the output
+ * buffers; otherwise every slot is written to the same position.  The order
+ * is kept by a single-pass decoupled look-back scan: workgroups take tiles
+ * in increasing order from a ticket counter, publish their tile's survivor
+ * count, and read their predecessors' published (aggregate or inclusive)
+ * counts.  Each published word is (epoch << 34) | (flag << 32) | count with
+ * flag 1 = aggregate, 2 = inclusive prefix; the epoch identifies the launch,
+ * so the status array never needs clearing.  The slot count before the
+ * block is read from the device (M_in) and the count after it is written
+ * there (M_out): the host never needs it to launch the next block. */
+struct FusedLaunch {
+    const float *freq_in;
+    const gf_mutid *ids_in;
+    float *freq_out;
+    gf_mutid *ids_out;
+    int64_t pitch;
+    const int *M_in;
+    int *M_out;
+    unsigned long long *status;   /* one word per tile */
+    unsigned int *ticket;         /* zero before the launch */
+    int *error;                   /* set if a look-back wait exceeded its bound */
+    unsigned long long epoch;
+    int ntiles;                   /* tiles launched (upper bound of the needed ones) */
+    int new_in_block;
+    int compact;
+    int preserve;
+};
+
+constexpr unsigned long long FLAG_AGG = 1ull, FLAG_INC = 2ull;
+
+__device__ __forceinline__ unsigned long long status_word(unsigned long long epoch, unsigned long long flag, unsigned int v) {
+    return (epoch << 34) | (flag << 32) | (unsigned long long)v;
+}
+
+template <int P, typename Sel>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_advance_fused(const FusedLaunch F, const SegLaunch<P> S,
+                                                                                          const Sel sel) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    __shared__ int sh_tile;
+    __shared__ unsigned int sh_wcount[WG / 64];
+    __shared__ unsigned int sh_excl;
+    __shared__ unsigned long long sh_live[64];
+    if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
+    const int M0 = *F.M_in;
+    const int Mend = M0 + F.new_in_block;
+    const int need_tiles = (Mend + WG - 1) / WG;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        /* without compaction every slot is kept in place */
+        if (Mend == 0 || !F.compact) *F.M_out = Mend;
+        if (need_tiles > F.ntiles) *F.error = 2;   /* host bound too small: slots would be dropped */
+    }
+    uint64_t cnt = 0;
+    const GenRow<P> *rows = S.rows;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_tile = (int)atomicAdd(F.ticket, 1u);
+        __syncthreads();
+        const int tile = sh_tile;
+        if (tile >= F.ntiles || tile >= need_tiles) break;
+        const int li = tile * WG + (int)threadIdx.x;
+        const bool valid = li < Mend;
+        float x[P];
+        int ogen = 0, opop = 0, born = -1;
+        uint32_t tid = 0;
+        bool preserved = false;
+#pragma unroll
+        for (int p = 0; p < P; p++) x[p] = 0.0f;
+        gf_mutid id;
+        id.origin_generation = 0; id.origin_population = 0; id.origin_threadID = 0; id.reserved = 0;
+        if (valid) {
+            if (li < M0) {
+#pragma unroll
+                for (int p = 0; p < P; p++) x[p] = F.freq_in[p * F.pitch + li];
+                id = F.ids_in[li];
+                ogen = id.origin_generation;
+                opop = id.origin_population;
+                preserved = id.origin_threadID < 0;
+                tid = (uint32_t)(preserved ? -id.origin_threadID : id.origin_threadID);
+            } else {
+                const int rel = li - M0 + S.cum_base;
+                int tb, pb;
+                find_birth<P>(rows, S.K, rel, tb, pb);
+                born = tb;
+                ogen = S.g0 + 1 + tb;
+                opop = pb;
+                tid = (uint32_t)(rel - rows[tb].newcum[pb] + 1);
+#pragma unroll
+                for (int p = 0; p < P; p++) x[p] = (p == pb) ? rows[tb].newfreq[p] : 0.0f;
+                id.origin_generation = ogen;
+                id.origin_population = opop;
+                id.origin_threadID = (int32_t)tid;
+                id.reserved = 0;
+            }
+        }
+        u32x4 rc[P];
+        bool keep = false;
+        const auto *crows = as_const(rows);
+        for (int t = 0; t < S.K; t++) {
+            const auto &R = crows[t];
+            const int g = S.g0 + 1 + t;
+            if (valid && born < t) {
+                const bool refresh = ((g & 3) == 0) || (t == born + 1);
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel);
+            }
+            keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
+            const uint64_t b = __ballot(keep);
+            if (lane == t) cnt += (uint64_t)__popcll(b);
+        }
+        if (!F.compact) {
+            if (valid) {
+#pragma unroll
+                for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + li] = x[p];
+                F.ids_out[li] = id;
+            }
+            continue;
+        }
+        /* stable compaction of this tile */
+        const uint64_t kb = __ballot(keep);
+        if (lane == 0) sh_wcount[w] = (unsigned int)__popcll(kb);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned int c = 0;
+            for (int k = 0; k < WG / 64; k++) c += sh_wcount[k];
+            unsigned int excl = 0;
+            if (tile == 0) {
+                __hip_atomic_store(&F.status[0], status_word(F.epoch, FLAG_INC, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&F.status[tile], status_word(F.epoch, FLAG_AGG, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int p = tile - 1;
+                long spins = 0;
+                while (p >= 0) {
+                    const unsigned long long v = __hip_atomic_load(&F.status[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long flag = (v >> 32) & 3ull;
+                    if ((v >> 34) != F.epoch || flag == 0ull) {
+                        if (++spins > (1l << 26)) { *F.error = 1; break; }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    excl += (unsigned int)v;
+                    if (flag == FLAG_INC) break;
+                    p--;
+                }
+                __hip_atomic_store(&F.status[tile], status_word(F.epoch, FLAG_INC, excl + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            sh_excl = excl;
+            if (tile == need_tiles - 1) *F.M_out = (int)(excl + c);
+        }
+        __syncthreads();
+        if (keep) {
+            unsigned int pos = sh_excl;
+            for (int k = 0; k < w; k++) pos += sh_wcount[k];
+            pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(kb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)kb, 0u));
+#pragma unroll
+            for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + pos] = x[p];
+            if (F.preserve) {
+                const int z = id.origin_threadID;
+                id.origin_threadID = -(z < 0 ? -z : z);
+            }
+            F.ids_out[pos] = id;
+        }
+    }
+    /* per-generation live counts: lane t of every wave holds generation t */
+    if (cnt) atomicAdd(&sh_live[lane], (unsigned long long)cnt);
+    __syncthreads();
+    if ((int)threadIdx.x < S.K && sh_live[threadIdx.x])
+        atomicAdd(&S.live[(size_t)(S.t0 + threadIdx.x) * 64 + (blockIdx.x & 63)], sh_live[threadIdx.x]);
+}
+
+/* ---------------- selection at the equilibrium start ----------------
+ * The frequencies at which the mutation-selection-equilibrium start
+ * evaluates selection (generation 0 of the run): the end points a, a - 1/Nc
+ * of trapezoid id (reference calculate_area / mse_integrand,
+ * go_fish_impl.cuh:24-53) and the class frequency (id + 1) / Nc (reference
+ * initialize_mse_frequency_array, :58-71), in the library's float order. */
+__host__ __device__ __forceinline__ void mse_area_points(int id, int Nc, float &a, float &b) {
+    GF_STRICT_FP
+    const float step = 1.0f / (float)Nc;
+    a = 1.0f - (float)id * step;
+    b = a - step;
+}
+
+__host__ __device__ __forceinline__ float mse_class_point(int id, int Nc) {
+    GF_STRICT_FP
+    return ((float)id + 1.0f) / (float)Nc;
+}
+
+/* Table of a caller's selection functor at those points, for one
+ * population with Nc chromosomes:
+ *   tab[id]          = s(a_id),        id in [0, Nc)
+ *   tab[Nc + id]     = s(a_id - 1/Nc), id in [0, Nc)
+ *   tab[2 Nc + id]   = s((id + 1)/Nc), id in [0, Nc - 1) */
+template <typename F>
+__global__ void k_mse_sel_table(const F f, int pop, int gen, int Nc, float *__restrict__ tab) {
+    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < Nc; id += gridDim.x * blockDim.x) {
+        float a, b;
+        mse_area_points(id, Nc, a, b);
+        tab[id] = f(pop, gen, a);
+        tab[Nc + id] = f(pop, gen, b);
+        if (id < Nc - 1) tab[2 * Nc + id] = f(pop, gen, mse_class_point(id, Nc));
+    }
+}
+
+} /* namespace gf */
+
+#endif /* GF_DRIFT_HPP_ */

@@ -16,6 +16,11 @@
 #include <stdint.h>
 
 #define GF_HD __host__ __device__ __forceinline__
+/* first statement of every floating-point function here: no FMA
+ * contraction whatever -ffp-contract the including translation unit uses
+ * (hipcc's default honours this pragma), so device results equal the host
+ * code's and the CPU oracle's bit for bit */
+#define GF_STRICT_FP _Pragma("clang fp contract(off)")
 
 namespace gf {
 
@@ -30,12 +35,14 @@ GF_HD uint32_t f2bits(float f) { return __builtin_bit_cast(uint32_t, f); }
 struct u32x4 { uint32_t x, y, z, w; };
 
 GF_HD void mulhilo(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo) {
+    GF_STRICT_FP
     uint64_t p = (uint64_t)a * (uint64_t)b;
     hi = (uint32_t)(p >> 32);
     lo = (uint32_t)p;
 }
 
 GF_HD u32x4 philox10(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
+    GF_STRICT_FP
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
@@ -50,11 +57,13 @@ GF_HD u32x4 philox10(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_
 }
 
 GF_HD uint32_t lane_of(const u32x4 &r, int i) {
+    GF_STRICT_FP
     return i == 0 ? r.x : (i == 1 ? r.y : (i == 2 ? r.z : r.w));
 }
 
 /* u * 2^-32 + 2^-33 in (0, 1] (reference shared.cuh:59-64) */
 GF_HD float u01(uint32_t u) {
+    GF_STRICT_FP
     float f = (float)u;
     f = f * 2.3283064365386963e-10f;
     return f + 1.1641532182693481e-10f;
@@ -64,6 +73,7 @@ GF_HD float u01(uint32_t u) {
 GF_HD float pow2i(int k) { return bits2f((uint32_t)(k + 127) << 23); }
 
 GF_HD float scale2(float p, int n) {
+    GF_STRICT_FP
     if (n > 127) return (p * pow2i(127)) * pow2i(n - 127);
     if (n < -126) return (p * pow2i(n + 100)) * pow2i(-100);
     return p * pow2i(n);
@@ -72,6 +82,7 @@ GF_HD float scale2(float p, int n) {
 /* exp(x): Cody-Waite split of ln 2 (0.693359375 + -2.12194440e-4), degree-7
  * Taylor polynomial on |r| <= ln2/2, exact scaling by 2^n. */
 GF_HD float exp_f(float x) {
+    GF_STRICT_FP
     if (x != x) return x;
     if (x > 88.72283935546875f) return __builtin_inff();
     if (x < -103.97208404541015625f) return 0.0f;
@@ -93,6 +104,7 @@ GF_HD float exp_f(float x) {
 /* log(x): x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
  * s = (m-1)/(m+1), odd series through s^9. */
 GF_HD float log_f(float x) {
+    GF_STRICT_FP
     if (x != x || x < 0.0f) return __builtin_nanf("");
     if (x == 0.0f) return -__builtin_inff();
     if (x == __builtin_inff()) return x;
@@ -117,6 +129,7 @@ GF_HD float log_f(float x) {
 /* log_f without its special cases: the same operations, so the same value,
  * for finite normal x > 0 */
 GF_HD float log_f_mid(float x) {
+    GF_STRICT_FP
     const uint32_t b = f2bits(x);
     int e = (int)((b >> 23) & 0xffu) - 127;
     float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
@@ -134,6 +147,7 @@ GF_HD float log_f_mid(float x) {
 }
 
 GF_HD float pow_f(float a, float b) {
+    GF_STRICT_FP
     if (a == 1.0f || b == 0.0f) return 1.0f;
     if (a == 0.0f) return b > 0.0f ? 0.0f : __builtin_inff();
     return exp_f(b * log_f(a));
@@ -142,6 +156,7 @@ GF_HD float pow_f(float a, float b) {
 /* inverse standard normal CDF, Wichura AS 241 PPND7 (Appl. Statist. 37,
  * 1988, 477-484); u clamped to <= 1 - 2^-24 so the result stays finite */
 GF_HD float normcdfinv_f(float p) {
+    GF_STRICT_FP
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
     const bool central = __builtin_fabsf(q) <= 0.425f;
@@ -175,6 +190,7 @@ GF_HD float normcdfinv_f(float p) {
 }
 
 GF_HD float round_away(float v) {
+    GF_STRICT_FP
     float t = __builtin_truncf(v);
     float d = v - t;
     if (__builtin_fabsf(d) >= 0.5f) t = t + (v < 0.0f ? -1.0f : 1.0f);
@@ -185,6 +201,7 @@ GF_HD float round_away(float v) {
  * <= -2^31 -> INT_MIN), truncating toward zero otherwise.  On the device
  * this is exactly v_cvt_i32_f32's defined behaviour. */
 GF_HD int f2i_sat(float v) {
+    GF_STRICT_FP
 #if defined(__HIP_DEVICE_COMPILE__)
     int r;
     asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
@@ -201,6 +218,7 @@ GF_HD int f2i_sat(float v) {
  * search stops once mean <= exit_mean(j); -1 = no early exit there
  * (reference shared.cuh:105-163 and :188-246) */
 GF_HD int exit_mean(int j) {
+    GF_STRICT_FP
     switch (j) {
     case 11: return 1;  case 14: return 2;  case 17: return 3;  case 20: return 4;
     case 22: return 5;  case 24: return 6;  case 26: return 7;  case 28: return 8;
@@ -247,6 +265,7 @@ static constexpr signed char kExitIndex[33] = {
     45, 46, 48, 50, 51, 53, 55, 56, 58, 59, 61, 62, 64, 65, 67, 68};
 
 GF_HD int exit_index(float mean) {
+    GF_STRICT_FP
     if (!(mean <= 32.0f)) return 70;
     int k = (int)__builtin_ceilf(mean);
     k = k < 1 ? 1 : k;
@@ -258,6 +277,7 @@ GF_HD int exit_index(float mean) {
 /* exp_f without its range checks: the same operations, so the same value,
  * for -87 <= x <= 88 (no NaN, no overflow, the scale 2^n stays normal) */
 GF_HD float exp_f_mid(float x) {
+    GF_STRICT_FP
     float t = x * 1.44269502162933349609375f;
     float n = __builtin_floorf(t + 0.5f);
     float r = x - n * 0.693359375f;
@@ -275,6 +295,7 @@ GF_HD float exp_f_mid(float x) {
 
 /* one term of the Poisson CDF recurrence; true if the inversion stops at it */
 GF_HD bool pois_term(float &emu, float &cdf, float mean, float rj, float r, int j, int jstop) {
+    GF_STRICT_FP
     emu = emu * (mean * rj);
     cdf = cdf + emu;
     return cdf >= r || j == jstop;
@@ -287,6 +308,7 @@ GF_HD bool pois_term(float &emu, float &cdf, float mean, float rj, float r, int 
  * loop stays rolled (lanes leave it as they finish, with no nested branch
  * per term) and takes four reciprocals per trip in one scalar load. */
 GF_HD int poiscdfinv(float r, float mean) {
+    GF_STRICT_FP
     float emu = exp_f_mid(-1.0f * mean);
     float cdf = emu;
     if (cdf >= r) return 0;
@@ -305,6 +327,7 @@ GF_HD int poiscdfinv(float r, float mean) {
 /* binomial CDF inversion (N < 100 only): kept out of line, it is off the
  * hot path */
 static __host__ __device__ __attribute__((noinline)) int binomcdfinv(float r, float mean, float x, float n) {
+    GF_STRICT_FP
     float emu = pow_f(1.0f - x, n);
     if (emu == 1.0f) emu = exp_f(-1.0f * mean);
     float cdf = emu;
@@ -324,6 +347,7 @@ static __host__ __device__ __attribute__((noinline)) int binomcdfinv(float r, fl
  * (or binomial, N < 100) tails share one CDF inversion; the rest is the
  * rounded normal approximation */
 GF_HD int binom_draw(uint32_t bits, float mean, float var, float N) {
+    GF_STRICT_FP
     const float u = u01(bits);
     const bool lo = mean <= 6.0f;
     const bool hi = !lo && (mean >= N - 6.0f);
@@ -337,6 +361,7 @@ GF_HD int binom_draw(uint32_t bits, float mean, float var, float N) {
 
 /* approximate Poisson draw (reference shared.cuh:251-256) */
 GF_HD int pois_draw(uint32_t bits, float mean, float var, float N) {
+    GF_STRICT_FP
     const float u = u01(bits);
     const bool lo = mean <= 6.0f;
     const bool hi = !lo && (mean >= N - 6.0f);
@@ -351,12 +376,14 @@ GF_HD int pois_draw(uint32_t bits, float mean, float var, float N) {
 
 /* effective number of chromosomes (int)(2N/(1+F)) (reference go_fish_impl.cuh:659) */
 GF_HD int nchrom(int N, float F) {
+    GF_STRICT_FP
     float v = (float)(2 * N) / (1.0f + F);
     return f2i_sat(__builtin_truncf(v));
 }
 
 /* equilibrium frequency weight (reference go_fish_impl.cuh:20-22) */
 GF_HD float mse_weight(float i, int N, float F, float h, float s) {
+    GF_STRICT_FP
     float A = ((2.0f * h) + ((1.0f - (2.0f * h)) * i)) * (1.0f - F) + (2.0f * F);
     float e = ((((float)(2 * N) * s) * i) * A) / (1.0f + F);
     return exp_f(e);
@@ -366,6 +393,7 @@ GF_HD float mse_weight(float i, int N, float F, float h, float s) {
  * generation g: {|threadID|, origin_generation, origin_pop<<16 | pop, g>>2};
  * the draw is lane g & 3. */
 GF_HD u32x4 drift_block(uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop, int pop, int g) {
+    GF_STRICT_FP
     return philox10(k0, k1, tid, (uint32_t)ogen, ((uint32_t)opop << 16) | (uint32_t)pop, (uint32_t)(g >> 2));
 }
 
@@ -373,6 +401,7 @@ GF_HD u32x4 drift_block(uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int op
  * (reference calc_new_mutations_Index, go_fish_impl.cuh:454-467; counter
  * {0, g, pop, 1}) */
 GF_HD int new_mutation_count(int Nc, float mu, float L, uint32_t k0, uint32_t k1, int g, int pop) {
+    GF_STRICT_FP
     if (Nc <= 0) return 0;
     float lambda = (mu * (float)Nc) * L;
     if (lambda == 0.0f) return 0;

@@ -126,6 +126,7 @@ static int check_params(const gf_sim_desc *d, int t, int g, const std::vector<ui
 template <int P>
 struct PlainRun {
     const gf_sim_desc *d;
+    const gf_selection_hooks *hooks = nullptr;   /* caller-compiled selection, or NULL: linear form */
     gf_result *R;
     hipStream_t st = nullptr;
     int G, start, final_g, interval;
@@ -205,7 +206,7 @@ struct PlainRun {
                 for (int k = 0; k < P; k++) r.mig[k][p] = mig[k * P + p];
                 r.newfreq[p] = Nc[p] > 0 ? 1.0f / (float)Nc[p] : 0.0f;
                 if (pg[p].N > 0) r.present_mask |= 1 << p;
-                if (pg[p].sel_slope == 0.0f && pg[p].sel_intercept == 0.0f) r.neutral_mask |= 1 << p;
+                if (!hooks && pg[p].sel_slope == 0.0f && pg[p].sel_intercept == 0.0f) r.neutral_mask |= 1 << p;
             }
             for (int p = 0; p < P; p++) {
                 r.newcum[p] = (int)cum;
@@ -257,13 +258,28 @@ struct PlainRun {
             s.Nc = Nc; s.Nind = pg.N; s.F = pg.F; s.h = pg.h; s.mu = pg.mu;
             s.sa = pg.sel_slope; s.sb = pg.sel_intercept; s.L = d->num_sites;
             s.k0 = k0; s.k1 = k1; s.pop = p;
-            s.neutral = (pg.sel_slope == 0.0f && pg.sel_intercept == 0.0f) ? 1 : 0;
+            s.neutral = (!hooks && pg.sel_slope == 0.0f && pg.sel_intercept == 0.0f) ? 1 : 0;
             s.preserve = pres ? 1 : 0;
             s.class_off = coff; s.area_off = aoff; s.chunk_off = choff;
             s.sim_base = 0;
             coff += Nc - 1; aoff += Nc; choff += (Nc + MSE_CHUNK - 1) / MSE_CHUNK;
             if (!s.neutral) any_sel = true;
             segs.push_back(s);
+        }
+        /* a caller-compiled selection functor: tabulate it at the
+         * equilibrium points of every population (generation start) */
+        DevBuf<float> d_seltab;
+        if (hooks && !segs.empty()) {
+            int64_t n = 0;
+            for (const MseSeg &m : segs) n += 3 * (int64_t)m.Nc;
+            GF_TRY(d_seltab.alloc((size_t)n));
+            int64_t o = 0;
+            for (MseSeg &m : segs) {
+                const int32_t rc = hooks->mse_table(hooks->user, m.pop, start, m.Nc, d_seltab.p + o, (void *)st);
+                if (rc != GF_OK) { set_error("run_sim: selection table hook failed"); return rc; }
+                m.sel_tab = d_seltab.p + o;
+                o += 3 * (int64_t)m.Nc;
+            }
         }
         M = 0;
         if (segs.empty()) return ensure_capacity(1);
@@ -488,7 +504,12 @@ struct PlainRun {
         const int grid = std::max(1, std::min(ntiles, num_cus() * 8));
         hipEvent_t ev = nullptr;
         t_adv.begin(st, ev);
-        hipLaunchKernelGGL((k_advance_fused<P>), dim3(grid), dim3(WG), 0, st, F, sl);
+        if (hooks) {
+            const int32_t rc = hooks->advance(hooks->user, P, &F, &sl, grid, (void *)st);
+            if (rc != GF_OK) { set_error("run_sim: selection advance hook failed"); return rc; }
+        } else {
+            hipLaunchKernelGGL((k_advance_fused<P, LinearSel>), dim3(grid), dim3(WG), 0, st, F, sl, LinearSel());
+        }
         GF_HIP(hipGetLastError());
         t_adv.end(ev);
         freqA.swap(freqB);
@@ -636,9 +657,10 @@ struct PlainRun {
 };
 
 template <int P>
-static int run_p(const gf_sim_desc *d, gf_result *R) {
+static int run_p(const gf_sim_desc *d, const gf_selection_hooks *hooks, gf_result *R) {
     PlainRun<P> r;
     r.d = d;
+    r.hooks = hooks;
     r.R = R;
     return r.run();
 }
@@ -665,7 +687,7 @@ int32_t gf_device_count(int32_t *out) {
     return GF_OK;
 }
 
-int32_t gf_run(const gf_sim_desc *d, gf_result **out) {
+static int32_t run_any(const gf_sim_desc *d, const gf_selection_hooks *hooks, gf_result **out) {
     if (!out) { gf::set_error("gf_run: out is NULL"); return GF_ERR_ARG; }
     *out = nullptr;
     if (!d || !d->params || !d->migration || !d->preserve || !d->sample) {
@@ -684,18 +706,29 @@ int32_t gf_run(const gf_sim_desc *d, gf_result **out) {
     std::unique_ptr<gf_result> R(new gf_result);
     int rc;
     switch (d->num_populations) {
-    case 1: rc = gf::run_p<1>(d, R.get()); break;
-    case 2: rc = gf::run_p<2>(d, R.get()); break;
-    case 3: rc = gf::run_p<3>(d, R.get()); break;
-    case 4: rc = gf::run_p<4>(d, R.get()); break;
-    case 5: rc = gf::run_p<5>(d, R.get()); break;
-    case 6: rc = gf::run_p<6>(d, R.get()); break;
-    case 7: rc = gf::run_p<7>(d, R.get()); break;
-    default: rc = gf::run_p<8>(d, R.get()); break;
+    case 1: rc = gf::run_p<1>(d, hooks, R.get()); break;
+    case 2: rc = gf::run_p<2>(d, hooks, R.get()); break;
+    case 3: rc = gf::run_p<3>(d, hooks, R.get()); break;
+    case 4: rc = gf::run_p<4>(d, hooks, R.get()); break;
+    case 5: rc = gf::run_p<5>(d, hooks, R.get()); break;
+    case 6: rc = gf::run_p<6>(d, hooks, R.get()); break;
+    case 7: rc = gf::run_p<7>(d, hooks, R.get()); break;
+    default: rc = gf::run_p<8>(d, hooks, R.get()); break;
     }
     if (rc != GF_OK) return rc;
     *out = R.release();
     return GF_OK;
+}
+
+int32_t gf_run(const gf_sim_desc *d, gf_result **out) { return run_any(d, nullptr, out); }
+
+int32_t gf_run_with_selection(const gf_sim_desc *d, const gf_selection_hooks *hooks, gf_result **out) {
+    if (!hooks || !hooks->mse_table || !hooks->advance) {
+        gf::set_error("gf_run_with_selection: hooks or one of its functions is NULL");
+        if (out) *out = nullptr;
+        return GF_ERR_ARG;
+    }
+    return run_any(d, hooks, out);
 }
 
 int32_t gf_result_num_samples(const gf_result *r) { return r ? (int32_t)r->samples.size() : GF_ERR_ARG; }

@@ -17,7 +17,7 @@ GOLDEN = os.path.join(REPO, "tests", "golden")
 def test_examples_build():
     r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "examples")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
-    for exe in ("speed", "dadi", "compilation"):
+    for exe in ("speed", "dadi", "compilation", "custom"):
         assert os.access(os.path.join(BUILD, exe), os.X_OK)
 
 
@@ -91,3 +91,23 @@ def test_mixed_compiler_example(tmp_path):
     freqs = np.array([[float(x) for x in r.split("\t")[1:]] for r in rows[:500]])
     assert freqs.shape[1] == 4
     assert np.allclose(np.round(freqs * 1000), freqs * 1000, atol=1e-3)   # 1000 haploid chromosomes
+
+
+@pytest.mark.gpu
+def test_selection_functors_without_linear_form():
+    """A selection functor with only the reference's operator() runs through
+    gf_run_with_selection (the drift kernel compiled in the program with the
+    functor): a functor with the values of selection_constant or
+    selection_linear_frequency_dependent gives the same run bit for bit, on
+    one and two populations; a non-linear functor (s = -1 above frequency
+    1/2, which halves such a mutation every generation) keeps every final
+    frequency near or below 1/2 while the neutral run has some far above."""
+    out = run_example("custom")
+    rows = {l.split("\t")[0]: l.split("\t")[1:] for l in out.strip().splitlines()}
+    for key in ("constant", "linear", "two_pops"):
+        a, b, n = rows[key]
+        assert a == b, (key, rows[key])
+        assert int(n) > 1000, (key, n)
+    mx_sel, mx_neutral, n = rows["halving"]
+    assert float(mx_sel) < 0.6, rows["halving"]
+    assert float(mx_neutral) > 0.8, rows["halving"]
