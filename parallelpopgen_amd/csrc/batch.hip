@@ -257,11 +257,11 @@ struct BatchRun {
         GF_HIP(hipEventRecord(i1, st));
 
         const int kb = std::max(1, std::min({KMAX, interval > 0 ? interval : KMAX, generations_per_launch()}));
-        DevBuf<unsigned long long> d_live;
-        GF_TRY(d_live.alloc(1));
+        DevBuf<unsigned long long> d_live;   /* 64 partial counters, summed at the end */
+        GF_TRY(d_live.alloc(64));
         DevBuf<unsigned int> d_ticket;
         GF_TRY(d_ticket.alloc(1));
-        GF_HIP(hipMemsetAsync(d_live.p, 0, sizeof(unsigned long long), st));
+        GF_HIP(hipMemsetAsync(d_live.p, 0, 64 * sizeof(unsigned long long), st));
 
         /* device scratch */
         DevBuf<int> d_tile_seg, d_meta, d_active, d_tot;
@@ -434,8 +434,10 @@ struct BatchRun {
             g += K;
         }
         GF_HIP(hipStreamSynchronize(st));
+        unsigned long long live64[64];
+        GF_HIP(hipMemcpy(live64, d_live.p, sizeof live64, hipMemcpyDeviceToHost));
         unsigned long long live = 0;
-        GF_HIP(hipMemcpy(&live, d_live.p, sizeof live, hipMemcpyDeviceToHost));
+        for (int k = 0; k < 64; k++) live += live64[k];
         R->live_mutation_generations = (int64_t)live;
         float ims = 0;
         (void)hipEventElapsedTime(&ims, i0, i1);

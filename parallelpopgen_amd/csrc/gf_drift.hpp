@@ -193,7 +193,8 @@ __device__ __forceinline__ void find_birth(const GenRow<P> *rows, int K, int rel
  * Advance every slot of every segment by its K generations.
  * BATCH=false: one segment, per-generation live counts (lane t of each wave
  * accumulates generation t, KMAX <= 64).  BATCH=true: tile_seg maps tiles to
- * segments and only the total of live mutation-generations is counted. */
+ * segments and only the total of live mutation-generations is counted, in
+ * seg0.live[0..63] (summed by the host). */
 template <int P, bool BATCH>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_advance(float *__restrict__ freq, int64_t pitch,
                                                 gf_mutid *__restrict__ ids,
@@ -274,13 +275,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
 #pragma unroll
             for (int p = 0; p < P; p++) freq[p * pitch + slot] = x[p];
         }
-        if (BATCH) {
-            /* flush per tile: segments differ between tiles */
-            if (lane == 0 && cnt) atomicAdd(S.live, (unsigned long long)cnt);
-            cnt = 0;
-        }
     }
-    if (!BATCH) {
+    if (BATCH) {
+        /* every segment counts into the same 64 counters (seg0.live): one
+         * atomic per wave at the end, spread over 64 addresses (a per-tile
+         * atomic on one address held every wave at the next barrier) */
+        if (lane == 0 && cnt) atomicAdd(&seg0.live[blockIdx.x & 63], (unsigned long long)cnt);
+    } else {
         __shared__ unsigned long long sh[64];
         if (threadIdx.x < 64) sh[threadIdx.x] = 0;
         __syncthreads();
