@@ -600,7 +600,9 @@ struct PlainRun {
                 const int te = t + K;
                 const int g = start + te;
                 const bool pres = d->preserve[te] || d->sample[te];
-                const bool cmp = interval > 0 && (g == next_compact || g == final_g || pres);
+                /* with compaction at every launch the reference's compaction
+                 * generations need no launch boundary of their own */
+                const bool cmp = interval > 0 && ((g == next_compact && !compact_every_launch) || g == final_g || pres);
                 if (cmp || d->sample[te] || te == G || K == kpl) break;
             }
             const int te = t + K;
@@ -611,6 +613,7 @@ struct PlainRun {
             if (rc) break;
             t = te;
             if (user_cmp) next_compact = g + interval;
+            while (interval > 0 && next_compact <= g) next_compact += interval;
             if (d->sample[t] && g != final_g) {
                 int cur = 0;
                 if ((rc = exact_M(cur))) break;
