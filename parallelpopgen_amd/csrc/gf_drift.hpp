@@ -293,41 +293,70 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     }
 }
 
-/* ---------------- fused advance + stable compaction ----------------
- * One launch advances every slot of a segment by K generations and, when
- * `compact` is set, writes only the survivors (compaction keep rule at the
- * last generation of the block) in their original order into the output
- * buffers; otherwise every slot is written to the same position.  The order
- * is kept by a single-pass decoupled look-back scan: workgroups take tiles
- * in increasing order from a ticket counter, publish their tile's survivor
- * count, and read their predecessors' published (aggregate or inclusive)
- * counts.  Each published word is (epoch << 34) | (flag << 32) | count with
- * flag 1 = aggregate, 2 = inclusive prefix; the epoch identifies the launch,
- * so the status array never needs clearing.  The slot count before the
- * block is read from the device (M_in) and the count after it is written
- * there (M_out): the host never needs it to launch the next block. */
+/* ---------------- fused advance + tile-local compaction ----------------
+ * One launch advances every slot by K generations and writes the slots to
+ * keep, in their order, packed at the start of the slot's own output tile
+ * (tile i of the output holds its survivors in [256 i, 256 i + cnt_out[i])).
+ * No tile waits for another.  k_tile_scan then turns the counts into the
+ * exclusive prefix pre[] (the compacted index of each tile's first
+ * survivor), the slot count, and for every tile j of the next launch the
+ * input tile src0[j] that holds compacted index 256 j.  The next launch
+ * reads its compacted slot li from input tile s, offset li - pre[s], with s
+ * the last tile whose pre[s] <= li: the stable order of the reference's
+ * compaction (go_fish_impl.cuh:370-429) without a separate scatter pass.
+ * Samples and capacity growth first make the state dense again
+ * (k_densify). */
 struct FusedLaunch {
     const float *freq_in;
     const gf_mutid *ids_in;
     float *freq_out;
     gf_mutid *ids_out;
     int64_t pitch;
-    const int *M_in;
-    int *M_out;
-    unsigned long long *status;   /* one word per tile */
+    const int *M_in;              /* compacted slots in the input */
+    const int *pre_in;            /* fragmented input: exclusive prefix per input tile, [T_in + 1] */
+    const int *src0_in;           /* fragmented input: first input tile of each tile of this launch */
+    const int *T_in;              /* fragmented input: number of input tiles */
+    unsigned int *cnt_out;        /* survivors per output tile */
     unsigned int *ticket;         /* zero before the launch */
-    int *error;                   /* set if a look-back wait exceeded its bound */
-    unsigned long long epoch;
+    int *error;                   /* set if the host's tile bound was too small */
     int ntiles;                   /* tiles launched (upper bound of the needed ones) */
     int new_in_block;
+    int dense;                    /* input is dense: compacted slot li is slot li */
     int compact;
     int preserve;
 };
 
-constexpr unsigned long long FLAG_AGG = 1ull, FLAG_INC = 2ull;
+/* Slot of compacted index li (< M_in) in a fragmented input, for every lane
+ * of one tile.  sh_pre is a 257-entry LDS window pre[s0 .. s0 + 256] filled
+ * by the caller (INT_MAX past the end); s0 = the input tile holding the
+ * tile's first compacted index, so sh_pre[0] <= li for every lane. */
+__device__ __forceinline__ int64_t frag_slot(int li, int s0, const int *sh_pre, const int *__restrict__ pre, int T_in) {
+    int lo = 0, hi = WG;                 /* last k with sh_pre[k] <= li */
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sh_pre[mid] <= li) lo = mid; else hi = mid - 1;
+    }
+    int s = s0 + lo;
+    if (lo == WG) {                      /* more than 256 input tiles before li: search the rest */
+        int l2 = s, h2 = T_in - 1;
+        while (l2 < h2) {
+            const int mid = (l2 + h2 + 1) >> 1;
+            if (pre[mid] <= li) l2 = mid; else h2 = mid - 1;
+        }
+        s = l2;
+        return (int64_t)s * WG + (li - pre[s]);
+    }
+    return (int64_t)s * WG + (li - sh_pre[lo]);
+}
 
-__device__ __forceinline__ unsigned long long status_word(unsigned long long epoch, unsigned long long flag, unsigned int v) {
-    return (epoch << 34) | (flag << 32) | (unsigned long long)v;
+/* loads the LDS window of frag_slot for tile `tile` (all threads; ends with
+ * a barrier) */
+__device__ __forceinline__ int load_frag_window(int tile, const int *__restrict__ src0, const int *__restrict__ pre,
+                                                int T_in, int *sh_pre) {
+    const int s0 = src0[tile];
+    for (int k = threadIdx.x; k <= WG; k += WG) sh_pre[k] = (s0 + k <= T_in) ? pre[s0 + k] : 0x7fffffff;
+    __syncthreads();
+    return s0;
 }
 
 template <int P, typename Sel>
@@ -337,17 +366,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int w = threadIdx.x >> 6;
     __shared__ int sh_tile;
     __shared__ unsigned int sh_wcount[WG / 64];
-    __shared__ unsigned int sh_excl;
+    __shared__ int sh_pre[WG + 1];
     __shared__ unsigned long long sh_live[64];
     if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
     const int M0 = *F.M_in;
     const int Mend = M0 + F.new_in_block;
     const int need_tiles = (Mend + WG - 1) / WG;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        /* without compaction every slot is kept in place */
-        if (Mend == 0 || !F.compact) *F.M_out = Mend;
-        if (need_tiles > F.ntiles) *F.error = 2;   /* host bound too small: slots would be dropped */
-    }
+    const int T_in = F.dense ? 0 : *F.T_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && need_tiles > F.ntiles) *F.error = 2;   /* slots would be dropped */
     uint64_t cnt = 0;
     const GenRow<P> *rows = S.rows;
     for (;;) {
@@ -358,6 +384,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
         if (tile >= F.ntiles || tile >= need_tiles) break;
         const int li = tile * WG + (int)threadIdx.x;
         const bool valid = li < Mend;
+        int s0 = 0;
+        if (!F.dense && tile * WG < M0) s0 = load_frag_window(tile, F.src0_in, F.pre_in, T_in, sh_pre);
         float x[P];
         int ogen = 0, opop = 0, born = -1;
         uint32_t tid = 0;
@@ -368,9 +396,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
         id.origin_generation = 0; id.origin_population = 0; id.origin_threadID = 0; id.reserved = 0;
         if (valid) {
             if (li < M0) {
+                const int64_t src = F.dense ? (int64_t)li : frag_slot(li, s0, sh_pre, F.pre_in, T_in);
 #pragma unroll
-                for (int p = 0; p < P; p++) x[p] = F.freq_in[p * F.pitch + li];
-                id = F.ids_in[li];
+                for (int p = 0; p < P; p++) x[p] = F.freq_in[p * F.pitch + src];
+                id = F.ids_in[src];
                 ogen = id.origin_generation;
                 opop = id.origin_population;
                 preserved = id.origin_threadID < 0;
@@ -405,57 +434,29 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             const uint64_t b = __ballot(keep);
             if (lane == t) cnt += (uint64_t)__popcll(b);
         }
-        if (!F.compact) {
-            if (valid) {
-#pragma unroll
-                for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + li] = x[p];
-                F.ids_out[li] = id;
-            }
-            continue;
-        }
-        /* stable compaction of this tile */
+        /* without compaction every slot of the block stays */
+        if (!F.compact) keep = valid;
+        /* tile-local stable packing */
         const uint64_t kb = __ballot(keep);
         if (lane == 0) sh_wcount[w] = (unsigned int)__popcll(kb);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned int c = 0;
-            for (int k = 0; k < WG / 64; k++) c += sh_wcount[k];
-            unsigned int excl = 0;
-            if (tile == 0) {
-                __hip_atomic_store(&F.status[0], status_word(F.epoch, FLAG_INC, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_store(&F.status[tile], status_word(F.epoch, FLAG_AGG, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                int p = tile - 1;
-                long spins = 0;
-                while (p >= 0) {
-                    const unsigned long long v = __hip_atomic_load(&F.status[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const unsigned long long flag = (v >> 32) & 3ull;
-                    if ((v >> 34) != F.epoch || flag == 0ull) {
-                        if (++spins > (1l << 26)) { *F.error = 1; break; }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    excl += (unsigned int)v;
-                    if (flag == FLAG_INC) break;
-                    p--;
-                }
-                __hip_atomic_store(&F.status[tile], status_word(F.epoch, FLAG_INC, excl + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            sh_excl = excl;
-            if (tile == need_tiles - 1) *F.M_out = (int)(excl + c);
-        }
-        __syncthreads();
         if (keep) {
-            unsigned int pos = sh_excl;
+            unsigned int pos = 0;
             for (int k = 0; k < w; k++) pos += sh_wcount[k];
             pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(kb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)kb, 0u));
+            const int64_t o = (int64_t)tile * WG + pos;
 #pragma unroll
-            for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + pos] = x[p];
+            for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + o] = x[p];
             if (F.preserve) {
                 const int z = id.origin_threadID;
                 id.origin_threadID = -(z < 0 ? -z : z);
             }
-            F.ids_out[pos] = id;
+            F.ids_out[o] = id;
+        }
+        if (threadIdx.x == 0) {
+            unsigned int c = 0;
+            for (int k = 0; k < WG / 64; k++) c += sh_wcount[k];
+            F.cnt_out[tile] = c;
         }
     }
     /* per-generation live counts: lane t of every wave holds generation t */

@@ -11,6 +11,66 @@
 
 namespace gf {
 
+/* ---------------- prefix of the fused launch's tile counts ----------------
+ * One workgroup.  T = tiles of the launch (from the device slot count
+ * M_in + new); pre[s] = survivors of tiles before s, pre[T] = total =
+ * *M_out; for every tile j of the next launch (256 j < total) src0[j] = the
+ * tile holding compacted index 256 j. */
+static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *__restrict__ cnt, const int *__restrict__ M_in,
+                                                           int new_in_block, int *__restrict__ pre, int *__restrict__ src0,
+                                                           int *__restrict__ M_out, int *__restrict__ T_out) {
+    __shared__ int sh[1024];
+    const int T = (*M_in + new_in_block + WG - 1) / WG;
+    const int per = (T + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    const int e = b + per < T ? b + per : T;
+    int sum = 0;
+    for (int s = b; s < e; s++) sum += (int)cnt[s];
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - sum;   /* exclusive prefix of this thread's range */
+    for (int s = b; s < e; s++) {
+        const int c = (int)cnt[s];
+        pre[s] = run;
+        if (c > 0) {
+            for (int j = (run + WG - 1) / WG; j * WG <= run + c - 1; j++) src0[j] = s;
+        }
+        run += c;
+    }
+    if (threadIdx.x == 1023) {
+        pre[T] = sh[1023];
+        *M_out = sh[1023];
+        *T_out = T;
+    }
+}
+
+/* dense copy of a fragmented state: slot j of the output = compacted index j */
+template <int P>
+__global__ __launch_bounds__(WG) void k_densify(const float *__restrict__ freq_in, const gf_mutid *__restrict__ ids_in,
+                                                float *__restrict__ freq_out, gf_mutid *__restrict__ ids_out, int64_t pitch,
+                                                const int *__restrict__ pre, const int *__restrict__ src0,
+                                                const int *__restrict__ T_in, const int *__restrict__ M_in) {
+    __shared__ int sh_pre[WG + 1];
+    const int M = *M_in, T = *T_in;
+    for (int tile = blockIdx.x; tile * WG < M; tile += gridDim.x) {
+        __syncthreads();
+        const int s0 = load_frag_window(tile, src0, pre, T, sh_pre);
+        const int li = tile * WG + (int)threadIdx.x;
+        if (li < M) {
+            const int64_t src = frag_slot(li, s0, sh_pre, pre, T);
+#pragma unroll
+            for (int p = 0; p < P; p++) freq_out[p * pitch + li] = freq_in[p * pitch + src];
+            ids_out[li] = ids_in[src];
+        }
+    }
+}
+
 /* ---------------- compaction (stable stream compaction) ---------------- */
 
 /* flag pass: one bit per slot (wave64 ballot), one count per 256-slot tile

@@ -153,7 +153,14 @@ struct PlainRun {
     /* fused advance + compaction state */
     DevBuf<int> d_M;                    /* [2] slot counts, alternately in and out */
     int mcur = 0;                       /* d_M[mcur] = current slot count */
-    DevBuf<unsigned long long> d_status;
+    /* fragmented layout of the A buffers after a fused launch (tile i holds
+     * its survivors at its start): per-tile counts, and two generations of
+     * prefix / first-source-tile arrays and tile counts, selected by par */
+    bool frag = false;
+    int par = 0;
+    int64_t tiles_cap = 0;              /* entries of d_cnt and of each d_src0 half */
+    DevBuf<unsigned int> d_cnt;
+    DevBuf<int> d_pre, d_src0, d_T;
     DevBuf<unsigned int> d_tickets;     /* one ticket counter per launch */
     DevBuf<int> d_err;
     PinBuf<int> h_ring;                 /* slot count after each launch */
@@ -242,6 +249,29 @@ struct PlainRun {
         GF_TRY(freqB.alloc((size_t)P * ncap));
         GF_TRY(idsB.alloc((size_t)ncap));
         cap = ncap;
+        /* the A state is dense here: the tile arrays hold nothing live */
+        const int64_t nt = ncap / WG;
+        if (nt > tiles_cap) {
+            GF_TRY(d_cnt.alloc((size_t)nt));
+            GF_TRY(d_pre.alloc(2 * (size_t)(nt + 1)));
+            GF_TRY(d_src0.alloc(2 * (size_t)nt));
+            tiles_cap = nt;
+        }
+        if (!d_T.p) GF_TRY(d_T.alloc(2));
+        return GF_OK;
+    }
+
+    /* make the A state dense (slot j = compacted index j) */
+    int densify() {
+        if (!frag) return GF_OK;
+        const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((M_bound + WG - 1) / WG, (int64_t)num_cus() * 8));
+        hipLaunchKernelGGL((k_densify<P>), dim3((unsigned)nb), dim3(WG), 0, st, freqA.p, idsA.p, freqB.p, idsB.p, cap,
+                           d_pre.p + (size_t)par * (tiles_cap + 1), d_src0.p + (size_t)par * tiles_cap, d_T.p + par,
+                           d_M.p + mcur);
+        GF_HIP(hipGetLastError());
+        freqA.swap(freqB);
+        idsA.swap(idsB);
+        frag = false;
         return GF_OK;
     }
 
@@ -420,6 +450,7 @@ struct PlainRun {
     }
 
     int take_sample(int t, bool with_ids) {
+        GF_TRY(densify());
         gf_sample_rec s;
         s.num_mutations = M;
         s.generation = start + t;
@@ -456,6 +487,7 @@ struct PlainRun {
      * count read from the device) move along */
     int grow(int64_t need) {
         if (need <= cap && freqA.p) return GF_OK;
+        GF_TRY(densify());
         int cur = 0;
         GF_TRY(exact_M(cur));
         M = cur;
@@ -475,21 +507,20 @@ struct PlainRun {
             bound = M_bound + new_in_block;
         }
         const int ntiles = (int)((bound + WG - 1) / WG);
-        if ((size_t)ntiles > d_status.n) {
-            GF_TRY(d_status.alloc((size_t)ntiles + (size_t)ntiles / 2 + 1024));
-            GF_HIP(hipMemsetAsync(d_status.p, 0, d_status.n * sizeof(unsigned long long), st));
-        }
+        const size_t pre_half = (size_t)tiles_cap + 1, src_half = (size_t)tiles_cap;
         FusedLaunch F{};
         F.freq_in = freqA.p; F.ids_in = idsA.p; F.freq_out = freqB.p; F.ids_out = idsB.p;
         F.pitch = cap;
         F.M_in = d_M.p + mcur;
-        F.M_out = d_M.p + (mcur ^ 1);
-        F.status = d_status.p;
+        F.pre_in = d_pre.p + par * pre_half;
+        F.src0_in = d_src0.p + par * src_half;
+        F.T_in = d_T.p + par;
+        F.cnt_out = d_cnt.p;
         F.ticket = d_tickets.p + launches;
         F.error = d_err.p;
-        F.epoch = (unsigned long long)(launches + 1);
         F.ntiles = ntiles;
         F.new_in_block = (int)new_in_block;
+        F.dense = frag ? 0 : 1;
         F.compact = compact ? 1 : 0;
         F.preserve = preserve ? 1 : 0;
         SegLaunch<P> sl{};
@@ -512,9 +543,16 @@ struct PlainRun {
         }
         GF_HIP(hipGetLastError());
         t_adv.end(ev);
+        /* survivors per tile -> prefix, slot count, next launch's source tiles */
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, d_cnt.p, d_M.p + mcur, (int)new_in_block,
+                           d_pre.p + (par ^ 1) * pre_half, d_src0.p + (par ^ 1) * src_half, d_M.p + (mcur ^ 1),
+                           d_T.p + (par ^ 1));
+        GF_HIP(hipGetLastError());
         freqA.swap(freqB);
         idsA.swap(idsB);
         mcur ^= 1;
+        par ^= 1;
+        frag = true;
         /* pipelined exact count: copy this launch's slot count to the host
          * and tighten the bound with the previous launch's count */
         const int slot = (int)(launches % RING);
@@ -629,7 +667,7 @@ struct PlainRun {
             M = cur;
             int err = 0;
             GF_HIP(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
-            if (err) { set_error("advance kernel: look-back wait exceeded its bound"); return GF_ERR_DEVICE; }
+            if (err) { set_error("advance kernel: the launch had fewer tiles than slots (internal bound)"); return GF_ERR_DEVICE; }
         }
         GF_TRY(take_sample(G, true));
         GF_HIP(hipStreamSynchronize(st));
