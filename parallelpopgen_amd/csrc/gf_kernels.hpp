@@ -18,35 +18,70 @@ namespace gf {
  * tile holding compacted index 256 j. */
 static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *__restrict__ cnt, const int *__restrict__ M_in,
                                                            int new_in_block, int *__restrict__ pre, int *__restrict__ src0,
-                                                           int *__restrict__ M_out, int *__restrict__ T_out) {
-    __shared__ int sh[1024];
+                                                           int *__restrict__ M_out, int *__restrict__ T_out,
+                                                           int *M_host) {
+    /* 16 counts per thread and round (four 16-byte loads), wave scans by
+     * shuffles, one LDS exchange of the 16 wave totals */
+    __shared__ int sh_w[16];
     const int T = (*M_in + new_in_block + WG - 1) / WG;
-    const int per = (T + 1023) / 1024;
-    const int b = threadIdx.x * per;
-    const int e = b + per < T ? b + per : T;
-    int sum = 0;
-    for (int s = b; s < e; s++) sum += (int)cnt[s];
-    sh[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int run = sh[threadIdx.x] - sum;   /* exclusive prefix of this thread's range */
-    for (int s = b; s < e; s++) {
-        const int c = (int)cnt[s];
-        pre[s] = run;
-        if (c > 0) {
-            for (int j = (run + WG - 1) / WG; j * WG <= run + c - 1; j++) src0[j] = s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < T; base += 16 * 1024) {
+        const int b = base + (int)threadIdx.x * 16;
+        unsigned int v[16];
+        if (b + 16 <= T) {
+            const uint4 *q = reinterpret_cast<const uint4 *>(cnt + b);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint4 u = q[k];
+                v[4 * k] = u.x; v[4 * k + 1] = u.y; v[4 * k + 2] = u.z; v[4 * k + 3] = u.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) v[i] = b + i < T ? cnt[b + i] : 0u;
         }
-        run += c;
+        int sum = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) sum += (int)v[i];
+        int x = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) sh_w[w] = x;
+        __syncthreads();
+        if (w == 0) {
+            int t = lane < 16 ? sh_w[lane] : 0;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const int y = __shfl_up(t, off);
+                if (lane >= off) t += y;
+            }
+            if (lane < 16) sh_w[lane] = t;
+        }
+        __syncthreads();
+        int run = carry + (w > 0 ? sh_w[w - 1] : 0) + x - sum;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int idx = b + i;
+            if (idx < T) {
+                const int c = (int)v[i];
+                pre[idx] = run;
+                if (c > 0) {
+                    for (int j = (run + WG - 1) / WG; j * WG <= run + c - 1; j++) src0[j] = idx;
+                }
+                run += c;
+            }
+        }
+        carry += sh_w[15];
+        __syncthreads();
     }
-    if (threadIdx.x == 1023) {
-        pre[T] = sh[1023];
-        *M_out = sh[1023];
+    if (threadIdx.x == 0) {
+        pre[T] = carry;
+        *M_out = carry;
         *T_out = T;
+        if (M_host) *M_host = carry;   /* pinned host word: the host reads it after the launch's event */
     }
 }
 

@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
+#include <cstring>
 #include <string>
 #include <vector>
 #include <chrono>
@@ -80,17 +82,36 @@ struct PinBuf {
     PinBuf(const PinBuf &) = delete;
     PinBuf &operator=(const PinBuf &) = delete;
     ~PinBuf() { if (p) (void)hipHostFree(p); }
-    int alloc(size_t count) {
+    int alloc(size_t count, unsigned int flags = hipHostMallocDefault) {
         if (count <= n && p) return GF_OK;
         if (p) (void)hipHostFree(p);
         p = nullptr;
         if (count == 0) count = 1;
-        hipError_t e = hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void **)&p, count * sizeof(T), flags);
         if (e != hipSuccess) { p = nullptr; set_error("hipHostMalloc failed"); return GF_ERR_ALLOC; }
         n = count;
         return GF_OK;
     }
 };
+
+/* Device -> host copy through a process-wide pinned staging buffer (kept
+ * between calls, grown on demand): a copy into pageable memory runs at a
+ * fraction of the link's bandwidth.  Serialised by a mutex. */
+inline int readback(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    static std::mutex mu;
+    static PinBuf<char> stage;
+    if (bytes == 0) return GF_OK;
+    std::lock_guard<std::mutex> lock(mu);
+    const size_t chunk = (size_t)64 << 20;
+    if (stage.n < std::min(bytes, chunk)) GF_TRY(stage.alloc(std::min(bytes, chunk)));
+    for (size_t off = 0; off < bytes; off += stage.n) {
+        const size_t b = std::min(stage.n, bytes - off);
+        GF_HIP(hipMemcpyAsync(stage.p, (const char *)src + off, b, hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        std::memcpy((char *)dst + off, stage.p, b);
+    }
+    return GF_OK;
+}
 
 /* HIP-event based kernel timer (on the library's stream) */
 struct KernelTimer {

@@ -24,13 +24,22 @@ struct gf_sample_rec {
     int32_t num_mutations = 0, generation = 0;
     std::vector<int32_t> Nchrom_e;
     std::vector<uint8_t> extinct;
-    std::vector<float> freq;   /* [P][M] */
+    std::vector<float> freq;   /* [P][M]; empty when the sample is the device-resident final state */
+    bool on_device = false;
 };
 
 struct gf_result {
     int32_t P = 0, G = 0;
     std::vector<gf_sample_rec> samples;
     std::vector<gf_mutid> ids;
+    int32_t num_ids = 0;
+    /* the final state stays in device memory until read: the final
+     * sample's frequencies ([P] rows of pitch dev_pitch) and the IDs are
+     * copied once, straight into the caller's buffers */
+    gf::DevBuf<float> dev_freq;
+    gf::DevBuf<gf_mutid> dev_ids;
+    int64_t dev_pitch = 0;
+    int dev_device = 0;
     std::vector<int64_t> live;
     gf_timing timing{};
 };
@@ -449,18 +458,24 @@ struct PlainRun {
         return GF_OK;
     }
 
-    int take_sample(int t, bool with_ids) {
+    int take_sample(int t, bool final_state) {
         GF_TRY(densify());
         gf_sample_rec s;
         s.num_mutations = M;
         s.generation = start + t;
-        s.freq.resize((size_t)P * M);
-        if (M > 0)
-            GF_HIP(hipMemcpy2DAsync(s.freq.data(), (size_t)M * sizeof(float), freqA.p, cap * sizeof(float),
-                                    (size_t)M * sizeof(float), P, hipMemcpyDeviceToHost, st));
-        if (with_ids) {
-            R->ids.resize(M);
-            if (M > 0) GF_HIP(hipMemcpyAsync(R->ids.data(), idsA.p, (size_t)M * sizeof(gf_mutid), hipMemcpyDeviceToHost, st));
+        if (final_state) {
+            /* the run's last use of the state: hand the buffers to the result */
+            GF_HIP(hipStreamSynchronize(st));
+            R->dev_freq.swap(freqA);
+            R->dev_ids.swap(idsA);
+            R->dev_pitch = cap;
+            R->num_ids = M;
+            GF_HIP(hipGetDevice(&R->dev_device));
+            s.on_device = true;
+        } else {
+            s.freq.resize((size_t)P * M);
+            for (int p = 0; p < P && M > 0; p++)
+                GF_TRY(readback(s.freq.data() + (size_t)p * M, freqA.p + (size_t)p * cap, (size_t)M * sizeof(float), st));
         }
         GF_HIP(hipStreamSynchronize(st));
         const gf_pop_gen *pg = d->params + (size_t)t * P;
@@ -544,19 +559,18 @@ struct PlainRun {
         GF_HIP(hipGetLastError());
         t_adv.end(ev);
         /* survivors per tile -> prefix, slot count, next launch's source tiles */
+        const int slot = (int)(launches % RING);
         hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, d_cnt.p, d_M.p + mcur, (int)new_in_block,
                            d_pre.p + (par ^ 1) * pre_half, d_src0.p + (par ^ 1) * src_half, d_M.p + (mcur ^ 1),
-                           d_T.p + (par ^ 1));
+                           d_T.p + (par ^ 1), &h_ring.p[slot]);
         GF_HIP(hipGetLastError());
         freqA.swap(freqB);
         idsA.swap(idsB);
         mcur ^= 1;
         par ^= 1;
         frag = true;
-        /* pipelined exact count: copy this launch's slot count to the host
-         * and tighten the bound with the previous launch's count */
-        const int slot = (int)(launches % RING);
-        GF_HIP(hipMemcpyAsync(&h_ring.p[slot], d_M.p + mcur, sizeof(int), hipMemcpyDeviceToHost, st));
+        /* pipelined exact count: the scan wrote this launch's slot count to
+         * pinned host memory; tighten the bound with the previous launch's */
         GF_HIP(hipEventRecord(ev_ring[slot], st));
         if (launches > 0) {
             const int prev = (int)((launches - 1) % RING);
@@ -617,7 +631,7 @@ struct PlainRun {
         GF_HIP(hipMemsetAsync(d_tickets.p, 0, ((size_t)G + 2) * sizeof(unsigned int), st));
         GF_TRY(d_err.alloc(1));
         GF_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), st));
-        GF_TRY(h_ring.alloc(RING + 1));
+        GF_TRY(h_ring.alloc(RING + 1, hipHostMallocCoherent));
         ev_ring.assign(RING, nullptr);
         for (auto &e : ev_ring) GF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         GF_HIP(hipStreamSynchronize(st));
@@ -786,17 +800,35 @@ int32_t gf_result_sample_info(const gf_result *r, int32_t s, int32_t *M, int32_t
     return GF_OK;
 }
 
+/* device -> host copy of a result's device-resident final state */
+static int32_t result_readback(const gf_result *r, void *dst, const void *src, size_t bytes) {
+    int cur = 0;
+    GF_HIP(hipGetDevice(&cur));
+    if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
+    const int rc = gf::readback(dst, src, bytes, nullptr);
+    if (cur != r->dev_device) (void)hipSetDevice(cur);
+    return rc;
+}
+
 int32_t gf_result_sample_freq(const gf_result *r, int32_t s, float *out) {
     if (!r || !out || s < 0 || s >= (int32_t)r->samples.size()) { gf::set_error("sample index out of range"); return GF_ERR_ARG; }
     const gf_sample_rec &x = r->samples[s];
-    if (!x.freq.empty()) memcpy(out, x.freq.data(), x.freq.size() * sizeof(float));
+    if (x.on_device) {
+        const size_t M = (size_t)x.num_mutations;
+        for (int p = 0; p < r->P && M > 0; p++)
+            GF_TRY(result_readback(r, out + (size_t)p * M, r->dev_freq.p + (size_t)p * r->dev_pitch, M * sizeof(float)));
+    } else if (!x.freq.empty()) {
+        memcpy(out, x.freq.data(), x.freq.size() * sizeof(float));
+    }
     return GF_OK;
 }
 
-int32_t gf_result_num_ids(const gf_result *r) { return r ? (int32_t)r->ids.size() : GF_ERR_ARG; }
+int32_t gf_result_num_ids(const gf_result *r) { return r ? r->num_ids : GF_ERR_ARG; }
 
 int32_t gf_result_ids(const gf_result *r, gf_mutid *out) {
     if (!r || !out) return GF_ERR_ARG;
+    if (r->num_ids > 0 && r->dev_ids.p)
+        return result_readback(r, out, r->dev_ids.p, (size_t)r->num_ids * sizeof(gf_mutid));
     if (!r->ids.empty()) memcpy(out, r->ids.data(), r->ids.size() * sizeof(gf_mutid));
     return GF_OK;
 }
