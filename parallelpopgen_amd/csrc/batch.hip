@@ -278,6 +278,7 @@ struct BatchRun {
         std::vector<int> layout;   /* simulations currently in the A buffers */
         for (int s = 0; s < S; s++) if (G[s] >= 0) layout.push_back(s);
         std::vector<int> Mend(M);
+        bool flags_ready = false;   /* d_masks / d_tcount hold the current layout's flags */
         int g = 0;
         std::vector<char> finished(S, 0);
         for (;;) {
@@ -287,25 +288,26 @@ struct BatchRun {
             const int nl = (int)layout.size();
             const int na = (int)active.size();
             const int K = kb;
-            /* ---- 1. segmented flag over the current layout ---- */
-            std::vector<int> tile_seg, tile_first(nl), tile_num(nl);
+            /* ---- 1. segmented flag over the current layout (the previous
+             * block's advance already wrote the flags; only the first block
+             * runs the flag pass) ---- */
+            std::vector<int> tile_first(nl), tile_num(nl);
+            int ntiles = 0;
             for (int k = 0; k < nl; k++) {
                 const int s = layout[k];
-                tile_first[k] = (int)tile_seg.size();
+                tile_first[k] = ntiles;
                 const int nt = (Mend[s] + WG - 1) / WG;
                 tile_num[k] = nt;
-                for (int q = 0; q < nt; q++) tile_seg.push_back(k);
+                ntiles += nt;
             }
-            const int ntiles = (int)tile_seg.size();
-            /* pinned host staging: [tile_seg | tile_first | tile_num | seg_M | present | preserve | total | active | tot] */
-            const size_t nmeta = (size_t)ntiles + 6 * (size_t)nl + 2 * (size_t)na + 8;
+            /* pinned host staging: [tile_first | tile_num | seg_M | present | preserve | total | active | tot] */
+            const size_t nmeta = 6 * (size_t)nl + 2 * (size_t)na + 8;
             GF_TRY(h_i.alloc(nmeta));
             GF_TRY(d_meta.alloc(nmeta));
             int *hm = h_i.p;
-            int *h_tile_seg = hm, *h_tile_first = hm + ntiles, *h_tile_num = h_tile_first + nl, *h_M = h_tile_num + nl,
+            int *h_tile_first = hm, *h_tile_num = h_tile_first + nl, *h_M = h_tile_num + nl,
                 *h_present = h_M + nl, *h_pres = h_present + nl, *h_total = h_pres + nl, *h_active = h_total + nl,
                 *h_tot = h_active + na;
-            for (int q = 0; q < ntiles; q++) h_tile_seg[q] = tile_seg[q];
             for (int k = 0; k < nl; k++) {
                 const int s = layout[k];
                 h_tile_first[k] = tile_first[k];
@@ -316,7 +318,7 @@ struct BatchRun {
                 h_total[k] = 0;
             }
             for (int a = 0; a < na; a++) { h_active[a] = active[a]; h_tot[a] = 0; }
-            const ptrdiff_t o_tile_first = ntiles, o_tile_num = o_tile_first + nl, o_M = o_tile_num + nl,
+            const ptrdiff_t o_tile_first = 0, o_tile_num = o_tile_first + nl, o_M = o_tile_num + nl,
                             o_present = o_M + nl, o_pres = o_present + nl, o_total = o_pres + nl,
                             o_active = o_total + nl, o_tot = o_active + na;
             GF_TRY(h_64.alloc(2 * (size_t)nl + 2));
@@ -324,15 +326,20 @@ struct BatchRun {
             for (int k = 0; k < nl; k++) h_64.p[k] = base[layout[k]];
             GF_HIP(hipMemcpyAsync(d_meta.p, hm, nmeta * sizeof(int), hipMemcpyHostToDevice, st));
             GF_HIP(hipMemcpyAsync(d_base64.p, h_64.p, (size_t)nl * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            GF_TRY(d_masks.alloc((size_t)std::max(ntiles, 1) * (WG / 64)));
-            GF_TRY(d_tcount.alloc((size_t)std::max(ntiles, 1)));
+            /* the advance sized d_masks / d_tcount for exactly these tiles
+             * and filled them: no reallocation in between */
+            if (!flags_ready) {
+                GF_TRY(d_masks.alloc((size_t)std::max(ntiles, 1) * (WG / 64)));
+                GF_TRY(d_tcount.alloc((size_t)std::max(ntiles, 1)));
+            }
             GF_TRY(d_toff.alloc((size_t)std::max(ntiles, 1)));
             hipEvent_t ce = nullptr;
             t_cmp.begin(st, ce);
             if (ntiles > 0) {
-                hipLaunchKernelGGL(k_flag<1>, dim3(ntiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p, d_base64.p,
-                                   d_meta.p + o_M, d_meta.p + o_present, d_meta.p, d_meta.p + o_tile_first,
-                                   d_masks.p, d_tcount.p);
+                if (!flags_ready)
+                    hipLaunchKernelGGL(k_flag<1>, dim3(ntiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
+                                       d_base64.p, d_meta.p + o_M, d_meta.p + o_present, d_meta.p + o_tile_first, nl,
+                                       d_masks.p, d_tcount.p);
                 hipLaunchKernelGGL(k_scan_tiles, dim3(nl), dim3(1024), 0, st, d_tcount.p, d_meta.p + o_tile_first,
                                    d_meta.p + o_tile_num, d_toff.p, d_meta.p + o_total);
                 GF_HIP(hipGetLastError());
@@ -366,7 +373,7 @@ struct BatchRun {
             if (ntiles > 0) {
                 hipLaunchKernelGGL(k_scatter<1>, dim3(ntiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p, freqB.p,
                                    (int64_t)freqB.n, idsB.p, d_base64.p, d_base64.p + nl, d_meta.p + o_pres,
-                                   d_meta.p, d_meta.p + o_tile_first, d_masks.p, d_toff.p);
+                                   d_meta.p + o_tile_first, nl, d_masks.p, d_toff.p);
                 GF_HIP(hipGetLastError());
             }
             t_cmp.end(ce);
@@ -387,7 +394,8 @@ struct BatchRun {
             /* ---- 5. one advance launch over every live simulation ---- */
             GF_TRY(h_sl.alloc(na));
             GF_TRY(d_sl.alloc(na));
-            std::vector<int> atile_seg, atile_first(na);
+            std::vector<int> atile_first(na);
+            int ntl = 0;
             for (int a = 0; a < na; a++) {
                 const int s = active[a];
                 SegLaunch<1> &L = h_sl.p[a];
@@ -403,25 +411,25 @@ struct BatchRun {
                 L.t0 = 0;
                 L.k0 = sims[s].k0;
                 L.k1 = sims[s].k1;
-                atile_first[a] = (int)atile_seg.size();
-                const int nt = (L.Mend + WG - 1) / WG;
-                for (int q = 0; q < nt; q++) atile_seg.push_back(a);
+                atile_first[a] = ntl;
+                ntl += (L.Mend + WG - 1) / WG;
                 Mend[s] = L.Mend;
             }
-            const int ntl = (int)atile_seg.size();
-            GF_TRY(d_tile_seg.alloc((size_t)ntl + na + 1));
-            std::vector<int> tmeta(atile_seg);
-            tmeta.insert(tmeta.end(), atile_first.begin(), atile_first.end());
+            GF_TRY(d_tile_seg.alloc((size_t)na + 1));
             GF_HIP(hipMemcpyAsync(d_sl.p, h_sl.p, na * sizeof(SegLaunch<1>), hipMemcpyHostToDevice, st));
-            GF_HIP(hipMemcpyAsync(d_tile_seg.p, tmeta.data(), tmeta.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            GF_HIP(hipMemcpyAsync(d_tile_seg.p, atile_first.data(), (size_t)na * sizeof(int), hipMemcpyHostToDevice, st));
+            /* the advance writes the next block's compaction flags */
+            GF_TRY(d_masks.alloc((size_t)std::max(ntl, 1) * (WG / 64)));
+            GF_TRY(d_tcount.alloc((size_t)std::max(ntl, 1)));
+            flags_ready = true;
             if (ntl > 0) {
                 const int grid = std::min(ntl, num_cus() * 8);
                 GF_HIP(hipMemsetAsync(d_ticket.p, 0, sizeof(unsigned int), st));
                 hipEvent_t ae = nullptr;
                 t_adv.begin(st, ae);
                 hipLaunchKernelGGL((k_advance<1, true>), dim3(grid), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
-                                   h_sl.p[0], (const SegLaunch<1> *)d_sl.p, (const int *)d_tile_seg.p,
-                                   (const int *)(d_tile_seg.p + ntl), ntl, d_ticket.p);
+                                   h_sl.p[0], (const SegLaunch<1> *)d_sl.p, (const int *)d_tile_seg.p, na, ntl,
+                                   d_ticket.p, d_masks.p, d_tcount.p);
                 GF_HIP(hipGetLastError());
                 t_adv.end(ae);
             }

@@ -186,6 +186,17 @@ __device__ __forceinline__ void find_birth(const GenRow<P> *rows, int K, int rel
     for (int q = 1; q < P; q++) if (rows[lo].newcum[q] <= rel) p = q;
 }
 
+/* segment of a tile: the last segment whose first tile is <= tile
+ * (segments without tiles share their first tile with the next one) */
+__device__ __forceinline__ int seg_of_tile(const int *__restrict__ tile_first, int nseg, int tile) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_first[mid] <= tile) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 /* Both drift kernels ask for 8 waves per SIMD (amdgpu_waves_per_eu): the
  * compiler then keeps them within 96 SGPRs (spilling a few to VGPR lanes),
  * which measured 5 % faster than 7 waves with no spills.
@@ -200,11 +211,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
                                                 gf_mutid *__restrict__ ids,
                                                 const SegLaunch<P> seg0,
                                                 const SegLaunch<P> *__restrict__ segs,
-                                                const int *__restrict__ tile_seg,
-                                                const int *__restrict__ tile_first,
-                                                int num_tiles, unsigned int *__restrict__ ticket) {
+                                                const int *__restrict__ tile_first, int nseg,
+                                                int num_tiles, unsigned int *__restrict__ ticket,
+                                                unsigned long long *__restrict__ masks,
+                                                unsigned int *__restrict__ tile_counts) {
     const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     __shared__ int sh_tile;
+    __shared__ unsigned int sh_wc[WG / 64];
     uint64_t cnt = 0;
     /* tiles are handed out dynamically (an atomic ticket): their cost varies
      * with the simulation they belong to */
@@ -214,7 +228,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
         __syncthreads();
         const int tile = sh_tile;
         if (tile >= num_tiles) break;
-        const int s = BATCH ? tile_seg[tile] : 0;
+        const int s = BATCH ? seg_of_tile(tile_first, nseg, tile) : 0;
         SegLaunch<P> S = seg0;
         if (BATCH) S = load_seg<P>(segs + s);
         const int li = (BATCH ? (tile - tile_first[s]) : tile) * WG + (int)threadIdx.x;
@@ -255,6 +269,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             }
         }
         u32x4 rc[P];
+        bool keep = false;
         const auto *crows = as_const(rows);
         for (int t = 0; t < S.K; t++) {
             const auto &R = crows[t];
@@ -263,12 +278,27 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
                 generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel());
             }
-            const bool keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
+            keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             const uint64_t b = __ballot(keep);
             if (BATCH) {
                 if (lane == 0) cnt += (uint64_t)__popcll(b);
             } else {
                 if (lane == t) cnt += (uint64_t)__popcll(b);
+            }
+        }
+        if (masks) {
+            /* the compaction flags of the block's last generation (what a
+             * flag pass over the written state would compute) */
+            const uint64_t kb = __ballot(keep);
+            if (lane == 0) {
+                masks[(size_t)tile * (WG / 64) + w] = kb;
+                sh_wc[w] = (unsigned int)__popcll(kb);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                unsigned int c = 0;
+                for (int k = 0; k < WG / 64; k++) c += sh_wc[k];
+                tile_counts[tile] = c;
             }
         }
         if (valid) {

@@ -116,12 +116,11 @@ __global__ __launch_bounds__(WG) void k_flag(const float *__restrict__ freq, int
                                              const int64_t *__restrict__ seg_base,
                                              const int *__restrict__ seg_M,
                                              const int *__restrict__ seg_present,
-                                             const int *__restrict__ tile_seg,
-                                             const int *__restrict__ tile_first,
+                                             const int *__restrict__ tile_first, int nseg,
                                              unsigned long long *__restrict__ masks,
                                              unsigned int *__restrict__ tile_counts) {
     const int tile = blockIdx.x;
-    const int s = tile_seg[tile];
+    const int s = seg_of_tile(tile_first, nseg, tile);
     const int li = (tile - tile_first[s]) * WG + (int)threadIdx.x;
     const int64_t slot = seg_base[s] + li;
     bool keep = false;
@@ -190,12 +189,11 @@ __global__ __launch_bounds__(WG) void k_scatter(const float *__restrict__ freq, 
                                                 const int64_t *__restrict__ seg_base,
                                                 const int64_t *__restrict__ seg_nbase,
                                                 const int *__restrict__ seg_preserve,
-                                                const int *__restrict__ tile_seg,
-                                                const int *__restrict__ tile_first,
+                                                const int *__restrict__ tile_first, int nseg,
                                                 const unsigned long long *__restrict__ masks,
                                                 const unsigned int *__restrict__ tile_off) {
     const int tile = blockIdx.x;
-    const int s = tile_seg[tile];
+    const int s = seg_of_tile(tile_first, nseg, tile);
     const int li = (tile - tile_first[s]) * WG + (int)threadIdx.x;
     const int w = threadIdx.x >> 6;
     const uint64_t b = masks[(size_t)tile * (WG / 64) + w];

@@ -151,7 +151,6 @@ struct PlainRun {
     /* compaction scratch */
     DevBuf<unsigned long long> d_masks;
     DevBuf<unsigned int> d_tcount, d_toff;
-    DevBuf<int> d_tile_seg;
     DevBuf<int> d_small;       /* tile_first, tile_num, seg_M, seg_present, seg_preserve, seg_total */
     DevBuf<int64_t> d_small64; /* seg_base, seg_nbase */
     PinBuf<int> h_small;
@@ -397,10 +396,8 @@ struct PlainRun {
     }
 
     int ensure_tiles(int64_t tiles) {
-        if ((size_t)tiles <= tile_seg_n && d_tile_seg.p) return GF_OK;
+        if ((size_t)tiles <= tile_seg_n && d_masks.p) return GF_OK;
         size_t n = std::max<size_t>((size_t)tiles, 1024);
-        GF_TRY(d_tile_seg.alloc(n));
-        GF_HIP(hipMemsetAsync(d_tile_seg.p, 0, n * sizeof(int), st));
         GF_TRY(d_masks.alloc(n * (WG / 64)));
         GF_TRY(d_tcount.alloc(n));
         GF_TRY(d_toff.alloc(n));
@@ -408,8 +405,8 @@ struct PlainRun {
         return GF_OK;
     }
 
-    /* count of slots the compaction rule keeps (initial live count) or the
-     * compaction itself (reference compact, go_fish_impl.cuh:370-429) */
+    /* count of the slots the compaction rule keeps (initial live count;
+     * reference flag_segregating_mutations, go_fish_impl.cuh:137-176) */
     int flag_scan(int present_mask, bool preserve, int &survivors) {
         const int64_t tiles = (M + WG - 1) / WG;
         GF_TRY(ensure_tiles(tiles));
@@ -429,7 +426,7 @@ struct PlainRun {
         GF_HIP(hipMemcpyAsync(d_small.p, hs, 6 * sizeof(int), hipMemcpyHostToDevice, st));
         GF_HIP(hipMemcpyAsync(d_small64.p, h_small64.p, 2 * sizeof(int64_t), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_flag<P>, dim3((unsigned)tiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
-                           d_small64.p, d_small.p + 2, d_small.p + 3, d_tile_seg.p, d_small.p + 0,
+                           d_small64.p, d_small.p + 2, d_small.p + 3, d_small.p + 0, 1,
                            d_masks.p, d_tcount.p);
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, d_tcount.p, d_small.p + 0, d_small.p + 1,
                            d_toff.p, d_small.p + 5);
@@ -437,24 +434,6 @@ struct PlainRun {
         GF_HIP(hipMemcpyAsync(&hs[8], d_small.p + 5, sizeof(int), hipMemcpyDeviceToHost, st));
         GF_HIP(hipStreamSynchronize(st));
         survivors = hs[8];
-        return GF_OK;
-    }
-
-    int compact(int present_mask, bool preserve) {
-        if (M == 0) return GF_OK;
-        hipEvent_t ev = nullptr;
-        t_cmp.begin(st, ev);
-        int surv = 0;
-        GF_TRY(flag_scan(present_mask, preserve, surv));
-        const int64_t tiles = (M + WG - 1) / WG;
-        hipLaunchKernelGGL(k_scatter<P>, dim3((unsigned)tiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
-                           freqB.p, (int64_t)cap, idsB.p, d_small64.p, d_small64.p + 1, d_small.p + 4,
-                           d_tile_seg.p, d_small.p + 0, d_masks.p, d_toff.p);
-        GF_HIP(hipGetLastError());
-        t_cmp.end(ev);
-        freqA.swap(freqB);
-        idsA.swap(idsB);
-        M = surv;
         return GF_OK;
     }
 
