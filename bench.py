@@ -62,11 +62,9 @@ def parse():
 def gpu_step(capi, scenarios, G):
     d = scenarios.plain_large(G=G)
     t0 = time.perf_counter()
-    r = capi.run(d, timing=True)
-    last = r["samples"][-1]
-    row = last["freq"][0]
-    capi.site_frequency_spectrum(row, 100, d.num_sites)
-    capi.site_frequency_spectrum(row, 1000, d.num_sites)
+    # the final generation's SFS at n = 100 and 1000, binned on the device
+    # before the state is copied out (gf_result_site_frequency_spectrum)
+    r = capi.run(d, timing=True, final_sfs=[(0, 100), (0, 1000)])
     dt = time.perf_counter() - t0
     mutgens = int(r["live"][1:].sum())
     return dt, mutgens, r

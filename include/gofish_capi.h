@@ -170,6 +170,33 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t num_mutations,
                                    int32_t sample_size, float num_sites,
                                    float *out, int32_t device);
 
+/* The same spectrum for time sample `sample`, population `population` of a
+ * gf_run result, computed where the sample lives: the final sample stays
+ * in device memory after gf_run, so it is binned there without a copy to
+ * the host and back (the reference copies every sample to the host,
+ * go_fish_impl.cuh:477-501, and back to the device for the spectrum,
+ * spectrum.cu:213-294).  Requires 1 <= sample_size < the population's
+ * chromosome count in that sample. */
+int32_t gf_result_site_frequency_spectrum(const gf_result *result, int32_t sample, int32_t population,
+                                          int32_t sample_size, float num_sites, float *out);
+
+/* Joint expected SFS of samples of n_a and n_b chromosomes from two
+ * populations (no reference counterpart: the reference's spectrum is per
+ * population, spectrum.h:23).  out[k_a * (n_b + 1) + k_b], k_a in
+ * [0, n_a], k_b in [0, n_b]: the sum over mutations of
+ * P(k_a | n_a, x_a) P(k_b | n_b, x_b) (a frequency of 0 or 1 puts all its
+ * mass at k = 0 or n); num_sites - num_mutations monomorphic sites are
+ * added at [0][0].  Each product is summed in 2^-32 fixed point, so the
+ * result does not depend on thread order.  freq_a, freq_b: host pointers,
+ * M values each (the two populations' rows of one time sample). */
+int32_t gf_joint_site_frequency_spectrum(const float *freq_a, const float *freq_b, int32_t num_mutations,
+                                         int32_t n_a, int32_t n_b, float num_sites, float *out, int32_t device);
+/* the same for populations pop_a, pop_b of time sample `sample` of a
+ * gf_run result, binned where the sample lives (see
+ * gf_result_site_frequency_spectrum) */
+int32_t gf_result_joint_site_frequency_spectrum(const gf_result *result, int32_t sample, int32_t pop_a, int32_t pop_b,
+                                                int32_t n_a, int32_t n_b, float num_sites, float *out);
+
 /* ----- batched simulation (no reference counterpart: replaces the
  * back-to-back run_sim + site_frequency_spectrum loop of
  * examples/example_dadi/run.cu:139-150) ----- */

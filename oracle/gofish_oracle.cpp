@@ -775,6 +775,48 @@ static void sfs(const float *freq, int M, int n, float num_sites, std::vector<fl
     for (int k = 1; k < n; k++) out[k] = (float)((double)acc[k] * sc);
 }
 
+/* per-population sample probabilities of one mutation, the float sequence
+ * of sfs() above: P(k | n, p) = exp(log C(n,k) + k log p + (n-k) log q),
+ * 0 where the exponent is below -23; p <= 0 or p >= 1: all mass at k = 0
+ * or k = n */
+static void sample_pmf(float pf, int n, const std::vector<float> &lc, int half_n, float *pmf) {
+    for (int k = 0; k <= n; k++) pmf[k] = 0.0f;
+    if (pf <= 0.0f) { pmf[0] = 1.0f; return; }
+    if (pf >= 1.0f) { pmf[n] = 1.0f; return; }
+    float lp = log_f(pf), lq = log_f(1.0f - pf);
+    for (int k = 0; k <= n; k++) {
+        float coeff = k < half_n ? lc[k] : lc[n - k];
+        float t = (coeff + (float)k * lp) + (float)(n - k) * lq;
+        if (t < -23.0f) continue;
+        pmf[k] = exp_f(t);
+    }
+}
+
+/* joint sample SFS of two populations (no reference counterpart; the
+ * reference's spectrum is per population, spectrum.h:23):
+ * out[ka*(nb+1)+kb] = sum over mutations of Pa(ka) Pb(kb), each product
+ * truncated to 2^-32 fixed point and summed exactly; monomorphic sites not
+ * in the array (num_sites - M) go to [0][0] */
+static void joint_sfs(const float *fa, const float *fb, int M, int na, int nb, float num_sites, std::vector<float> &out) {
+    std::vector<float> la, lb; int ha, hb;
+    log_binom_table(na, la, ha);
+    log_binom_table(nb, lb, hb);
+    const int B = (na + 1) * (nb + 1);
+    std::vector<uint64_t> acc(B, 0);
+    std::vector<float> pa(na + 1), pb(nb + 1);
+    for (int i = 0; i < M; i++) {
+        sample_pmf(fa[i], na, la, ha, pa.data());
+        sample_pmf(fb[i], nb, lb, hb, pb.data());
+        for (int a = 0; a <= na; a++) {
+            if (pa[a] == 0.0f) continue;
+            for (int b = 0; b <= nb; b++) acc[(size_t)a * (nb + 1) + b] += (uint64_t)((pa[a] * pb[b]) * 4294967296.0f);
+        }
+    }
+    out.assign(B, 0.0f);
+    const double sc = 1.0 / 4294967296.0;
+    for (int k = 0; k < B; k++) out[k] = (float)((double)acc[k] * sc + (k == 0 ? (double)num_sites - (double)M : 0.0));
+}
+
 } /* namespace orc */
 
 /* ===================== C API (test hooks) ===================== */
@@ -873,6 +915,14 @@ int32_t orc_population_histogram(const float *freq, int32_t M, int32_t Nc, float
     std::vector<uint32_t> h;
     pop_hist(freq, M, Nc, num_sites, h);
     for (int i = 0; i < Nc; i++) out[i] = (float)h[i];
+    return 0;
+}
+int32_t orc_joint_site_frequency_spectrum(const float *fa, const float *fb, int32_t M, int32_t na, int32_t nb,
+                                          float num_sites, float *out) {
+    if (na <= 0 || nb <= 0 || M < 0) return GF_ERR_ARG;
+    std::vector<float> o;
+    joint_sfs(fa, fb, M, na, nb, num_sites, o);
+    memcpy(out, o.data(), o.size() * sizeof(float));
     return 0;
 }
 int32_t orc_site_frequency_spectrum(const float *freq, int32_t M, int32_t n, float num_sites, float *out) {

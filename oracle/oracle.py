@@ -54,6 +54,7 @@ def load():
         "orc_result_free": (None, [vp]),
         "orc_population_histogram": (i32, [vp, i32, i32, f32, vp]),
         "orc_site_frequency_spectrum": (i32, [vp, i32, i32, f32, vp]),
+        "orc_joint_site_frequency_spectrum": (i32, [vp, vp, i32, i32, i32, f32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -198,4 +199,14 @@ def site_frequency_spectrum(freq_row, n, num_sites):
     f = np.ascontiguousarray(freq_row, np.float32)
     out = np.empty(int(n), np.float32)
     assert load().orc_site_frequency_spectrum(_p(f), len(f), int(n), float(num_sites), _p(out)) == 0
+    return out
+
+
+def joint_site_frequency_spectrum(freq_a, freq_b, n_a, n_b, num_sites):
+    a = np.ascontiguousarray(freq_a, np.float32)
+    b = np.ascontiguousarray(freq_b, np.float32)
+    assert len(a) == len(b)
+    out = np.empty((int(n_a) + 1, int(n_b) + 1), np.float32)
+    assert load().orc_joint_site_frequency_spectrum(_p(a), _p(b), len(a), int(n_a), int(n_b), float(num_sites),
+                                                    _p(out)) == 0
     return out

@@ -86,6 +86,11 @@ SIGNATURES = {
     "gf_batch_timing": (C.c_int32, [C.c_void_p, C.POINTER(gf_timing)]),
     "gf_batch_free": (None, [C.c_void_p]),
     "gf_run_with_selection": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gf_result_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p]),
+    "gf_joint_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                                     C.c_void_p, C.c_int32]),
+    "gf_result_joint_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                                            C.c_int32, C.c_float, C.c_void_p]),
     "gf_sg_create": (C.c_int32, [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     "gf_sg_free": (None, [C.c_void_p]),
     "gf_sg_num_params": (C.c_int32, [C.c_void_p]),
@@ -224,13 +229,31 @@ def check_any(rc, lib, prefix):
         raise GoFishError(rc, msg)
 
 
-def run(desc: SimDesc, timing=False):
-    """Runs a simulation on the GPU; returns samples, IDs, live counts (and timing)."""
+def run(desc: SimDesc, timing=False, final_sfs=(), final_joint_sfs=()):
+    """Runs a simulation on the GPU; returns samples, IDs, live counts (and
+    timing).  final_sfs: (population, sample_size) pairs whose sample SFS of
+    the final generation is computed on the device before the state is
+    copied to the host (out["final_sfs"][(population, n)]);
+    final_joint_sfs: (pop_a, pop_b, n_a, n_b) joint spectra, likewise
+    (out["final_sfs"][(pop_a, pop_b, n_a, n_b)])."""
     lib = load()
     h = C.c_void_p()
     check(lib.gf_run(C.byref(desc.desc), C.byref(h)))
     try:
+        sfs = {}
+        last = lib.gf_result_num_samples(h) - 1
+        for pop, n in final_sfs:
+            o = np.empty(int(n), np.float32)
+            check(lib.gf_result_site_frequency_spectrum(h, last, int(pop), int(n), float(desc.num_sites),
+                                                         o.ctypes.data_as(C.c_void_p)))
+            sfs[(int(pop), int(n))] = o
+        for pa, pb, na, nb in final_joint_sfs:
+            o = np.empty((int(na) + 1, int(nb) + 1), np.float32)
+            check(lib.gf_result_joint_site_frequency_spectrum(h, last, int(pa), int(pb), int(na), int(nb),
+                                                               float(desc.num_sites), o.ctypes.data_as(C.c_void_p)))
+            sfs[(int(pa), int(pb), int(na), int(nb))] = o
         out = read_result(lib, "gf_", h, desc.P, desc.G)
+        out["final_sfs"] = sfs
         if timing:
             t = gf_timing()
             lib.gf_result_timing(h, C.byref(t))
@@ -246,6 +269,20 @@ def population_frequency_histogram(freq_row, Nchrom_e, num_sites, device=-1):
     out = np.empty(int(Nchrom_e), np.float32)
     check(lib.gf_population_histogram(f.ctypes.data_as(C.c_void_p), len(f), int(Nchrom_e), float(num_sites),
                                       out.ctypes.data_as(C.c_void_p), device))
+    return out
+
+
+def joint_site_frequency_spectrum(freq_a, freq_b, n_a, n_b, num_sites, device=-1):
+    """joint expected SFS [(n_a+1), (n_b+1)] of two populations' rows"""
+    lib = load()
+    a = np.ascontiguousarray(freq_a, np.float32)
+    b = np.ascontiguousarray(freq_b, np.float32)
+    if a.shape != b.shape:
+        raise ValueError("the two rows must have the same length")
+    out = np.empty((int(n_a) + 1, int(n_b) + 1), np.float32)
+    check(lib.gf_joint_site_frequency_spectrum(a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p), len(a),
+                                               int(n_a), int(n_b), float(num_sites), out.ctypes.data_as(C.c_void_p),
+                                               device))
     return out
 
 

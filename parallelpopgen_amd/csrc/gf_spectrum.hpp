@@ -91,9 +91,88 @@ __global__ __launch_bounds__(WG) void k_sfs(const float *__restrict__ freq, cons
 
 void log_binom_table(int n, std::vector<float> &lc, int &half_n);
 void finish_sfs(const unsigned long long *acc, unsigned long long nonseg, int n, int M, float num_sites, float *out);
+/* sample SFS of one population row already in device memory (M floats) */
+int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st);
 int sfs_device(const float *d_freq, const int64_t *d_seg_base, const int *d_seg_M, const int *d_tile_seg,
                const int *d_tile_first, int num_tiles, int nseg, int n, unsigned long long *d_acc,
                hipStream_t st, DevBuf<float> &d_lc);
+
+/* ---------------- joint sample SFS of two populations ----------------
+ * (no reference counterpart: the reference's spectrum is per population,
+ * spectrum.h:23).  out[ka (nb+1) + kb] = sum over mutations of
+ * Pa(ka) Pb(kb), with the per-population sample probabilities of k_sfs
+ * (mass at k = 0 or n for a frequency of 0 or 1); each product is
+ * truncated to 2^-32 fixed point and summed exactly, so the result does
+ * not depend on order.  Output-stationary: a workgroup owns
+ * JS_PER_THREAD * 256 bins (grid y) and a range of JS_MUT_PER_WG mutations
+ * (grid x), streams them through LDS in batches with their probability
+ * vectors, and adds its sums to the global bins once. */
+constexpr int JS_PER_THREAD = 12;
+constexpr int JS_MUT_PER_WG = 2048;
+constexpr int JS_LDS_FLOATS = 16384;
+
+__device__ __forceinline__ float sample_prob(float pf, int n, int k, const float *__restrict__ lc, int half_n) {
+    GF_STRICT_FP
+    if (pf <= 0.0f) return k == 0 ? 1.0f : 0.0f;
+    if (pf >= 1.0f) return k == n ? 1.0f : 0.0f;
+    const float lp = log_f(pf), lq = log_f(1.0f - pf);
+    const float coeff = k < half_n ? lc[k] : lc[n - k];
+    const float t = (coeff + (float)k * lp) + (float)(n - k) * lq;
+    return t < -23.0f ? 0.0f : exp_f(t);
+}
+
+static __global__ __launch_bounds__(WG) void k_joint_sfs(const float *__restrict__ fa, const float *__restrict__ fb, int M,
+                                                  int na, int nb, int half_a, int half_b,
+                                                  const float *__restrict__ la, const float *__restrict__ lb,
+                                                  int batch, unsigned long long *__restrict__ acc) {
+    GF_STRICT_FP
+    extern __shared__ float sh[];
+    float *sa = sh;                          /* [batch][na + 1] */
+    float *sb = sh + (size_t)batch * (na + 1);   /* [batch][nb + 1] */
+    const int B = (na + 1) * (nb + 1);
+    const int bin0 = blockIdx.y * WG * JS_PER_THREAD;
+    unsigned long long a[JS_PER_THREAD];
+    int ka[JS_PER_THREAD], kb[JS_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < JS_PER_THREAD; j++) {
+        a[j] = 0;
+        const int bin = bin0 + (int)threadIdx.x + WG * j;
+        ka[j] = bin < B ? bin / (nb + 1) : -1;
+        kb[j] = bin < B ? bin % (nb + 1) : 0;
+    }
+    const int m_begin = blockIdx.x * JS_MUT_PER_WG;
+    const int m_end = m_begin + JS_MUT_PER_WG < M ? m_begin + JS_MUT_PER_WG : M;
+    for (int mb = m_begin; mb < m_end; mb += batch) {
+        const int nm = m_end - mb < batch ? m_end - mb : batch;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nm * (na + 1); idx += WG) {
+            const int m = idx / (na + 1), k = idx % (na + 1);
+            sa[idx] = sample_prob(fa[mb + m], na, k, la, half_a);
+        }
+        for (int idx = threadIdx.x; idx < nm * (nb + 1); idx += WG) {
+            const int m = idx / (nb + 1), k = idx % (nb + 1);
+            sb[idx] = sample_prob(fb[mb + m], nb, k, lb, half_b);
+        }
+        __syncthreads();
+        for (int m = 0; m < nm; m++) {
+            const float *pa = sa + (size_t)m * (na + 1);
+            const float *pb = sb + (size_t)m * (nb + 1);
+#pragma unroll
+            for (int j = 0; j < JS_PER_THREAD; j++) {
+                if (ka[j] < 0) continue;
+                const float x = pa[ka[j]];
+                if (x != 0.0f) a[j] += (unsigned long long)((x * pb[kb[j]]) * 4294967296.0f);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JS_PER_THREAD; j++)
+        if (ka[j] >= 0 && a[j]) atomicAdd(&acc[bin0 + (int)threadIdx.x + WG * j], a[j]);
+}
+
+/* joint sample SFS of two population rows already in device memory */
+int joint_sfs_of_device_rows(const float *d_fa, const float *d_fb, int M, int na, int nb, float num_sites, float *out,
+                             hipStream_t st);
 
 } /* namespace gf */
 

@@ -12,6 +12,7 @@
  */
 #include "gf_kernels.hpp"
 #include "gf_runtime.hpp"
+#include "gf_spectrum.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -800,6 +801,55 @@ int32_t gf_result_sample_freq(const gf_result *r, int32_t s, float *out) {
         memcpy(out, x.freq.data(), x.freq.size() * sizeof(float));
     }
     return GF_OK;
+}
+
+int32_t gf_result_site_frequency_spectrum(const gf_result *r, int32_t s, int32_t pop, int32_t n, float num_sites,
+                                          float *out) {
+    if (!r || !out || s < 0 || s >= (int32_t)r->samples.size() || pop < 0 || pop >= r->P || n <= 0) {
+        gf::set_error("site_frequency_spectrum: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    const gf_sample_rec &x = r->samples[s];
+    if (n >= x.Nchrom_e[pop]) {
+        gf::set_error("site_frequency_spectrum error: requested sample_size out of range [1,population_size): sample_size " + std::to_string(n) + " [1," + std::to_string(x.Nchrom_e[pop]) + ")");
+        return GF_ERR_ARG;
+    }
+    if (!x.on_device)
+        return gf_site_frequency_spectrum(x.freq.data() + (size_t)pop * x.num_mutations, x.num_mutations, n, num_sites,
+                                          out, r->dev_device);
+    int cur = 0;
+    GF_HIP(hipGetDevice(&cur));
+    if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int rc = gf::sfs_of_device_row(r->dev_freq.p + (size_t)pop * r->dev_pitch, x.num_mutations, n, num_sites, out, st);
+    (void)hipStreamDestroy(st);
+    if (cur != r->dev_device) (void)hipSetDevice(cur);
+    return rc;
+}
+
+int32_t gf_result_joint_site_frequency_spectrum(const gf_result *r, int32_t s, int32_t pa, int32_t pb, int32_t na,
+                                                int32_t nb, float num_sites, float *out) {
+    if (!r || !out || s < 0 || s >= (int32_t)r->samples.size() || pa < 0 || pa >= r->P || pb < 0 || pb >= r->P ||
+        na <= 0 || nb <= 0) {
+        gf::set_error("joint_site_frequency_spectrum: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    const gf_sample_rec &x = r->samples[s];
+    const int M = x.num_mutations;
+    if (!x.on_device)
+        return gf_joint_site_frequency_spectrum(x.freq.data() + (size_t)pa * M, x.freq.data() + (size_t)pb * M, M, na, nb,
+                                                num_sites, out, r->dev_device);
+    int cur = 0;
+    GF_HIP(hipGetDevice(&cur));
+    if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int rc = gf::joint_sfs_of_device_rows(r->dev_freq.p + (size_t)pa * r->dev_pitch,
+                                                r->dev_freq.p + (size_t)pb * r->dev_pitch, M, na, nb, num_sites, out, st);
+    (void)hipStreamDestroy(st);
+    if (cur != r->dev_device) (void)hipSetDevice(cur);
+    return rc;
 }
 
 int32_t gf_result_num_ids(const gf_result *r) { return r ? r->num_ids : GF_ERR_ARG; }

@@ -54,6 +54,64 @@ int sfs_device(const float *d_freq, const int64_t *d_seg_base, const int *d_seg_
     return GF_OK;
 }
 
+int joint_sfs_of_device_rows(const float *d_fa, const float *d_fb, int M, int na, int nb, float num_sites, float *out,
+                             hipStream_t st) {
+    if ((int64_t)(na + 1) * (nb + 1) > ((int64_t)1 << 26) || na + nb + 2 > JS_LDS_FLOATS) {
+        set_error("joint_site_frequency_spectrum: sample sizes too large");
+        return GF_ERR_UNSUPPORTED;
+    }
+    std::vector<float> la, lb;
+    int ha, hb;
+    log_binom_table(na, la, ha);
+    log_binom_table(nb, lb, hb);
+    const int B = (na + 1) * (nb + 1);
+    DevBuf<float> d_l;
+    DevBuf<unsigned long long> d_acc;
+    GF_TRY(d_l.alloc(la.size() + lb.size()));
+    GF_TRY(d_acc.alloc((size_t)B));
+    GF_HIP(hipMemcpyAsync(d_l.p, la.data(), la.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(d_l.p + la.size(), lb.data(), lb.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemsetAsync(d_acc.p, 0, (size_t)B * sizeof(unsigned long long), st));
+    if (M > 0) {
+        const int batch = std::max(1, std::min(32, JS_LDS_FLOATS / (na + nb + 2)));
+        const dim3 grid((unsigned)((M + JS_MUT_PER_WG - 1) / JS_MUT_PER_WG), (unsigned)((B + WG * JS_PER_THREAD - 1) / (WG * JS_PER_THREAD)));
+        const size_t lds = (size_t)batch * (na + nb + 2) * sizeof(float);
+        hipLaunchKernelGGL(k_joint_sfs, grid, dim3(WG), lds, st, d_fa, d_fb, M, na, nb, ha, hb, d_l.p,
+                           d_l.p + la.size(), batch, d_acc.p);
+        GF_HIP(hipGetLastError());
+    }
+    std::vector<unsigned long long> acc((size_t)B);
+    GF_HIP(hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    const double sc = 1.0 / 4294967296.0;
+    for (int k = 0; k < B; k++) out[k] = (float)((double)acc[k] * sc + (k == 0 ? (double)num_sites - (double)M : 0.0));
+    return GF_OK;
+}
+
+/* sample SFS of one population row already in device memory (M floats) */
+int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st) {
+    DevBuf<float> d_lc;
+    DevBuf<unsigned long long> d_acc;
+    DevBuf<int> d_meta;
+    DevBuf<int64_t> d_base;
+    const int tiles = (M + WG - 1) / WG;
+    GF_TRY(d_acc.alloc((size_t)n + 2));
+    GF_TRY(d_meta.alloc((size_t)tiles + 2));
+    GF_TRY(d_base.alloc(1));
+    std::vector<int> meta((size_t)tiles + 2, 0);
+    meta[tiles] = M;      /* seg_M */
+    meta[tiles + 1] = 0;  /* tile_first */
+    const int64_t base0 = 0;
+    GF_HIP(hipMemcpyAsync(d_meta.p, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(d_base.p, &base0, sizeof(int64_t), hipMemcpyHostToDevice, st));
+    GF_TRY(sfs_device(d_row, d_base.p, d_meta.p + tiles, d_meta.p, d_meta.p + tiles + 1, tiles, 1, n, d_acc.p, st, d_lc));
+    std::vector<unsigned long long> acc((size_t)n + 2);
+    GF_HIP(hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    finish_sfs(acc.data(), acc[n + 1], n, M, num_sites, out);
+    return GF_OK;
+}
+
 } /* namespace gf */
 
 using namespace gf;
@@ -114,35 +172,37 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t M, int32_t n, floa
     GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     int rc = GF_OK;
     {
-        DevBuf<float> d_f, d_lc;
-        DevBuf<unsigned long long> d_acc;
-        DevBuf<int> d_meta;
-        DevBuf<int64_t> d_base;
-        const int tiles = (M + WG - 1) / WG;
+        DevBuf<float> d_f;
         rc = d_f.alloc(std::max(M, 1));
-        if (!rc) rc = d_acc.alloc((size_t)n + 2);
-        if (!rc) rc = d_meta.alloc((size_t)tiles + 2);
-        if (!rc) rc = d_base.alloc(1);
-        if (!rc) {
-            std::vector<int> meta((size_t)tiles + 2, 0);
-            meta[tiles] = M;      /* seg_M */
-            meta[tiles + 1] = 0;  /* tile_first */
-            const int64_t base0 = 0;
-            hipError_t e = hipSuccess;
-            if (M > 0) e = hipMemcpyAsync(d_f.p, freq, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(d_meta.p, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(d_base.p, &base0, sizeof(int64_t), hipMemcpyHostToDevice, st);
+        if (!rc && M > 0) {
+            const hipError_t e = hipMemcpyAsync(d_f.p, freq, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
             if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = GF_ERR_DEVICE; }
-            if (!rc) rc = sfs_device(d_f.p, d_base.p, d_meta.p + tiles, d_meta.p, d_meta.p + tiles + 1, tiles, 1, n,
-                                     d_acc.p, st, d_lc);
-            std::vector<unsigned long long> acc((size_t)n + 2);
-            if (!rc) {
-                e = hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
-                if (e == hipSuccess) e = hipStreamSynchronize(st);
-                if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = GF_ERR_DEVICE; }
-            }
-            if (!rc) finish_sfs(acc.data(), acc[n + 1], n, M, num_sites, out);
         }
+        if (!rc) rc = sfs_of_device_row(d_f.p, M, n, num_sites, out, st);
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int32_t gf_joint_site_frequency_spectrum(const float *freq_a, const float *freq_b, int32_t M, int32_t n_a, int32_t n_b,
+                                         float num_sites, float *out, int32_t device) {
+    if (!out || n_a <= 0 || n_b <= 0 || M < 0 || (M > 0 && (!freq_a || !freq_b))) {
+        set_error("joint_site_frequency_spectrum: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    GF_TRY(select_device(device));
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = GF_OK;
+    {
+        DevBuf<float> d_f;
+        rc = d_f.alloc(2 * (size_t)std::max(M, 1));
+        if (!rc && M > 0) {
+            hipError_t e = hipMemcpyAsync(d_f.p, freq_a, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_f.p + M, freq_b, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = GF_ERR_DEVICE; }
+        }
+        if (!rc) rc = joint_sfs_of_device_rows(d_f.p, d_f.p + M, M, n_a, n_b, num_sites, out, st);
     }
     (void)hipStreamDestroy(st);
     return rc;

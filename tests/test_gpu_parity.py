@@ -154,3 +154,43 @@ def test_spectrum_matches_oracle(gpu, oracle):
         s_g = gpu.site_frequency_spectrum(row, n, d.num_sites)
         s_o = oracle.site_frequency_spectrum(row, n, d.num_sites)
         assert np.array_equal(s_g.view(np.uint32), s_o.view(np.uint32)), n
+
+
+def test_final_sfs_on_device_matches_host_path(gpu, oracle):
+    """gf_result_site_frequency_spectrum bins the device-resident final
+    state: identical to the spectrum of the copied-out sample (and to the
+    oracle's), for every population of a multi-population run."""
+    d = S.three_population(scale=0.1, L=2e8, G=300, preserve_every=0, sample_every=0)
+    pairs = [(0, 50), (1, 20), (2, 100)]
+    r = gpu.run(d, final_sfs=pairs)
+    last = r["samples"][-1]
+    for pop, n in pairs:
+        host = gpu.site_frequency_spectrum(last["freq"][pop], n, d.num_sites)
+        dev = r["final_sfs"][(pop, n)]
+        assert np.array_equal(dev.view(np.uint32), host.view(np.uint32)), (pop, n)
+        ref = oracle.site_frequency_spectrum(last["freq"][pop], n, d.num_sites)
+        assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32)), (pop, n)
+    with pytest.raises(RuntimeError):
+        gpu.run(d, final_sfs=[(0, 10 ** 7)])
+
+
+def test_joint_sfs_matches_oracle(gpu, oracle):
+    """joint two-population SFS: GPU = oracle bit for bit, from host rows and
+    from the device-resident final state; ragged counts and frequencies of 0
+    and 1 included"""
+    rng = np.random.default_rng(31)
+    for M, na, nb in [(0, 5, 5), (1, 3, 9), (4097, 20, 30), (30000, 100, 60)]:
+        fa = rng.random(M).astype(np.float32)
+        fb = rng.random(M).astype(np.float32)
+        if M > 10:
+            fa[:5] = 0.0
+            fb[3:9] = 1.0
+        g = gpu.joint_site_frequency_spectrum(fa, fb, na, nb, 1e6)
+        o = oracle.joint_site_frequency_spectrum(fa, fb, na, nb, 1e6)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (M, na, nb)
+    d = split_scenario(G=299)   # both populations alive at the end
+    r = gpu.run(d, final_joint_sfs=[(0, 1, 40, 30)])
+    last = r["samples"][-1]
+    j = r["final_sfs"][(0, 1, 40, 30)]
+    o = oracle.joint_site_frequency_spectrum(last["freq"][0], last["freq"][1], 40, 30, d.num_sites)
+    assert np.array_equal(j.view(np.uint32), o.view(np.uint32))

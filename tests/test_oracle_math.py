@@ -204,3 +204,29 @@ def test_f2i_saturates_and_truncates(oracle):
     rng = np.random.default_rng(3)
     y = rng.uniform(-2e9, 2e9, 100000).astype(np.float32)
     assert np.array_equal(oracle.f2i(y).astype(np.int64), np.trunc(y.astype(np.float64)).astype(np.int64))
+
+
+def test_joint_sfs_oracle_against_binomial_sums(oracle):
+    """joint SFS = sum of products of binomial sample probabilities (float64
+    brute force); monomorphic sites at [0][0]; summing over one population
+    gives that population's unfolded sample spectrum."""
+    from scipy.stats import binom
+    rng = np.random.default_rng(7)
+    M, na, nb, L = 400, 12, 7, 5000.0
+    fa = rng.random(M).astype(np.float32)
+    fb = rng.random(M).astype(np.float32)
+    fa[:20] = 0.0
+    fb[20:30] = 1.0
+    fa[30:35] = 1.0
+    j = oracle.joint_site_frequency_spectrum(fa, fb, na, nb, L)
+    pa = binom.pmf(np.arange(na + 1)[None, :], na, fa.astype(np.float64)[:, None])
+    pb = binom.pmf(np.arange(nb + 1)[None, :], nb, fb.astype(np.float64)[:, None])
+    ref = pa.T @ pb
+    ref[0, 0] += L - M
+    assert np.allclose(j, ref, rtol=2e-5, atol=2e-3)
+    assert abs(j.sum() - L) < 1e-2
+    # marginal over population b = population a's unfolded spectrum, which
+    # the marginal SFS folds (k = n_a and p in {0,1} into bin 0)
+    marg = j.sum(1).astype(np.float64)
+    m = oracle.site_frequency_spectrum(fa, na, L).astype(np.float64)
+    assert np.allclose(marg[1:na], m[1:na], rtol=1e-5, atol=1e-3)
