@@ -13,11 +13,18 @@
  *   two_pops  : the constant functor over two populations with migration      (bit for bit)
  *   halving   : s = -1 above frequency 1/2 (a mutation there is halved each
  *               generation): largest final frequency, and the neutral run's
+ *   joint     : Spectrum::joint_site_frequency_spectrum of the two-population
+ *               run (n = 30, 20): total mass over num_sites, and the largest
+ *               difference between its population-0 marginal and
+ *               Spectrum::site_frequency_spectrum (bins 1..29), relative to
+ *               max(1, bin)
  *   ./custom
  */
 #include "go_fish.cuh"
+#include "spectrum.h"
 #include <algorithm>
 #include <cstdint>
+#include <cmath>
 #include <cstring>
 #include <cstdio>
 
@@ -100,6 +107,20 @@ int main() {
         auto b = one(2, constant_functor{0.002f}, 200, 1e7f);
         std::printf("two_pops\t%016llx\t%016llx\t%d\n", (unsigned long long)digest(a), (unsigned long long)digest(b),
                     a.maximal_num_mutations());
+        const int s_last = a.num_time_samples() - 1;
+        Spectrum::SFS joint, marg;
+        Spectrum::joint_site_frequency_spectrum(joint, a, s_last, 0, 1, 30, 20);
+        Spectrum::site_frequency_spectrum(marg, a, s_last, 0, 30);
+        double total = 0.0, worst = 0.0;
+        for (int k = 0; k < 31 * 21; k++) total += joint.frequency_spectrum[k];
+        for (int ka = 1; ka < 30; ka++) {
+            double m = 0.0;
+            for (int kb = 0; kb <= 20; kb++) m += joint.frequency_spectrum[ka * 21 + kb];
+            /* relative for bins above one site, absolute below: products under 2^-32 truncate to 0 */
+            const double r = std::fabs(m - marg.frequency_spectrum[ka]) / std::max(1.0, (double)marg.frequency_spectrum[ka]);
+            worst = std::max(worst, r);
+        }
+        std::printf("joint\t%.7f\t%.3g\t%d\n", total / joint.num_sites, worst, joint.num_populations);
     }
     {
         auto a = one(1, halving_functor{}, 400, 2e8f);

@@ -35,6 +35,14 @@ void population_frequency_histogram(SFS &mySFS, const GO_Fish::allele_trajectori
 void site_frequency_spectrum(SFS &mySFS, const GO_Fish::allele_trajectories &all_results, const int sample_index,
                              const int population_index, const int sample_size, int cuda_device = -1);
 
+/** joint expected SFS of samples of sample_size_a and sample_size_b chromosomes from populations population_a and
+ *  population_b of time sample sample_index (no reference counterpart: the reference's spectrum is per
+ *  population).  num_populations = 2; frequency_spectrum has (sample_size_a + 1) * (sample_size_b + 1) bins,
+ *  bin [k_a][k_b] at k_a * (sample_size_b + 1) + k_b; bin [0][0] holds the sites monomorphic in both samples. */
+void joint_site_frequency_spectrum(SFS &mySFS, const GO_Fish::allele_trajectories &all_results, const int sample_index,
+                                   const int population_a, const int population_b, const int sample_size_a,
+                                   const int sample_size_b, int cuda_device = -1);
+
 } /* namespace Spectrum */
 
 #endif /* SPECTRUM_H_ */

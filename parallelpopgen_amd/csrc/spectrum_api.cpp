@@ -106,4 +106,34 @@ void site_frequency_spectrum(SFS &mySFS, const GO_Fish::allele_trajectories &all
     fill(mySFS, s, sample_size, population_index, v);
 }
 
+void joint_site_frequency_spectrum(SFS &mySFS, const GO_Fish::allele_trajectories &all_results, const int sample_index,
+                                   const int population_a, const int population_b, const int sample_size_a,
+                                   const int sample_size_b, int cuda_device) {
+    using namespace Spectrum_details;
+    const transfer_allele_trajectories::view va = transfer_allele_trajectories::get(all_results, sample_index, population_a);
+    const transfer_allele_trajectories::view vb = transfer_allele_trajectories::get(all_results, sample_index, population_b);
+    if (sample_size_a <= 0 || sample_size_a >= va.Nchrom_e || sample_size_b <= 0 || sample_size_b >= vb.Nchrom_e) {
+        std::fprintf(stderr, "joint_site_frequency_spectrum error: requested sample sizes out of range: sample_size %d [1,%d), sample_size %d [1,%d)\n",
+                     sample_size_a, va.Nchrom_e, sample_size_b, vb.Nchrom_e);
+        std::exit(1);
+    }
+    const size_t B = (size_t)(sample_size_a + 1) * (size_t)(sample_size_b + 1);
+    float *j = new float[B];
+    if (gf_joint_site_frequency_spectrum(va.row, vb.row, va.num_mutations, sample_size_a, sample_size_b, va.num_sites, j,
+                                         cuda_device) != GF_OK) {
+        std::fprintf(stderr, "%s\n", gf_last_error());
+        std::exit(1);
+    }
+    fill(mySFS, j, sample_size_a, population_a, va);
+    mySFS.num_populations = 2;
+    delete[] mySFS.sample_size;
+    delete[] mySFS.populations;
+    mySFS.sample_size = new int[2];
+    mySFS.sample_size[0] = sample_size_a;
+    mySFS.sample_size[1] = sample_size_b;
+    mySFS.populations = new int[2];
+    mySFS.populations[0] = population_a;
+    mySFS.populations[1] = population_b;
+}
+
 } /* namespace Spectrum */

@@ -101,13 +101,19 @@ def test_selection_functors_without_linear_form():
     selection_linear_frequency_dependent gives the same run bit for bit, on
     one and two populations; a non-linear functor (s = -1 above frequency
     1/2, which halves such a mutation every generation) keeps every final
-    frequency near or below 1/2 while the neutral run has some far above."""
+    frequency near or below 1/2 while the neutral run has some far above.
+    Also: the C++ joint two-population SFS of the two-population run sums
+    to num_sites and its population-0 marginal equals the C++ marginal SFS
+    (bins 1..n-1: 1e-3 of max(1, bin); each product is truncated to 2^-32,
+    at most (n_b + 1) M 2^-32 per bin)."""
     out = run_example("custom")
     rows = {l.split("\t")[0]: l.split("\t")[1:] for l in out.strip().splitlines()}
     for key in ("constant", "linear", "two_pops"):
         a, b, n = rows[key]
         assert a == b, (key, rows[key])
         assert int(n) > 1000, (key, n)
+    total, worst, npop = rows["joint"]
+    assert abs(float(total) - 1.0) < 1e-4 and float(worst) < 1e-3 and int(npop) == 2, rows["joint"]
     mx_sel, mx_neutral, n = rows["halving"]
     assert float(mx_sel) < 0.6, rows["halving"]
     assert float(mx_neutral) > 0.8, rows["halving"]
