@@ -258,7 +258,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_advance_fused (migration_selection_drift + add_new_mutations + tile-local compaction, 16 generations per launch)",
+            "kernel": "k_advance_fused (migration_selection_drift + add_new_mutations + tile-local compaction, "
+                      "%.1f generations per launch)" % (a.generations * a.steps / max(adv_launch, 1)),
             "avg_launch_ms": adv_ms / max(adv_launch, 1),
             "algorithmic_bytes_per_launch": BYTES_PER_MUTGEN * total_mg / max(adv_launch, 1),
         },

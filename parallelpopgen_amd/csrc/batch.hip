@@ -256,7 +256,7 @@ struct BatchRun {
         GF_TRY(init());
         GF_HIP(hipEventRecord(i1, st));
 
-        const int kb = std::max(1, std::min({KMAX, interval > 0 ? interval : KMAX, generations_per_launch()}));
+        const int kb = std::max(1, std::min({KMAX, interval > 0 ? interval : KMAX, generations_per_launch(16)}));
         DevBuf<unsigned long long> d_live;   /* 64 partial counters, summed at the end */
         GF_TRY(d_live.alloc(64));
         DevBuf<unsigned int> d_ticket;

@@ -406,12 +406,22 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     if (blockIdx.x == 0 && threadIdx.x == 0 && need_tiles > F.ntiles) *F.error = 2;   /* slots would be dropped */
     uint64_t cnt = 0;
     const GenRow<P> *rows = S.rows;
-    for (;;) {
+    /* Tile order.  One population: a fixed stride over the tiles (their
+     * costs are alike, and a ticket's atomic round trip was measured to cost
+     * more than the imbalance).  Several populations: an atomic ticket per
+     * tile (the cost of a tile varies with the populations its slots hold). */
+    const int last_tile = need_tiles < F.ntiles ? need_tiles : F.ntiles;
+    for (int it = 0;; it++) {
         __syncthreads();
-        if (threadIdx.x == 0) sh_tile = (int)atomicAdd(F.ticket, 1u);
-        __syncthreads();
-        const int tile = sh_tile;
-        if (tile >= F.ntiles || tile >= need_tiles) break;
+        int tile;
+        if constexpr (P == 1) {
+            tile = (int)blockIdx.x + it * (int)gridDim.x;
+        } else {
+            if (threadIdx.x == 0) sh_tile = (int)atomicAdd(F.ticket, 1u);
+            __syncthreads();
+            tile = sh_tile;
+        }
+        if (tile >= last_tile) break;
         const int li = tile * WG + (int)threadIdx.x;
         const bool valid = li < Mend;
         int s0 = 0;
@@ -476,12 +486,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(kb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)kb, 0u));
             const int64_t o = (int64_t)tile * WG + pos;
 #pragma unroll
-            for (int p = 0; p < P; p++) F.freq_out[p * F.pitch + o] = x[p];
+            for (int p = 0; p < P; p++) __builtin_nontemporal_store(x[p], &F.freq_out[p * F.pitch + o]);
             if (F.preserve) {
                 const int z = id.origin_threadID;
                 id.origin_threadID = -(z < 0 ? -z : z);
             }
-            F.ids_out[o] = id;
+            /* streamed out (the next launch reads them): non-temporal stores
+             * leave the L2 to this launch's inputs */
+            typedef int i4v __attribute__((ext_vector_type(4)));
+            const i4v v = {id.origin_generation, id.origin_population, id.origin_threadID, id.reserved};
+            __builtin_nontemporal_store(v, reinterpret_cast<i4v *>(&F.ids_out[o]));
         }
         if (threadIdx.x == 0) {
             unsigned int c = 0;

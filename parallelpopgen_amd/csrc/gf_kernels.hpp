@@ -85,6 +85,19 @@ static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *_
     }
 }
 
+/* live count of each generation: the sum of its 64 counters (the drift
+ * kernel spreads its atomics over 64 addresses per generation) */
+static __global__ __launch_bounds__(256) void k_sum_live(const unsigned long long *__restrict__ live,
+                                                         unsigned long long *__restrict__ out, int rows) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= rows) return;
+    const unsigned long long *r = live + (size_t)t * 64;
+    unsigned long long s = 0;
+#pragma unroll 8
+    for (int k = 0; k < 64; k++) s += r[k];
+    out[t] = s;
+}
+
 /* dense copy of a fragmented state: slot j of the output = compacted index j */
 template <int P>
 __global__ __launch_bounds__(WG) void k_densify(const float *__restrict__ freq_in, const gf_mutid *__restrict__ ids_in,

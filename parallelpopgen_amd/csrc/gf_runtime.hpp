@@ -20,7 +20,7 @@ namespace gf {
 void set_error(const std::string &msg);
 const std::string &get_error();
 bool timing_enabled();
-int generations_per_launch();
+int generations_per_launch(int dflt);
 
 struct Status {
     int code = GF_OK;

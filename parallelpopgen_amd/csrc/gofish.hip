@@ -53,8 +53,8 @@ void set_error(const std::string &m) { t_err = m; }
 const std::string &get_error() { return t_err; }
 static bool g_timing = false;
 bool timing_enabled() { return g_timing; }
-static int g_kpl = 16;   /* generations per advance launch (1..KMAX) */
-int generations_per_launch() { return g_kpl; }
+static int g_kpl = 0;    /* generations per advance launch (1..KMAX); 0: the caller's default */
+int generations_per_launch(int dflt) { return g_kpl > 0 ? g_kpl : dflt; }
 
 int select_device(int device) {
     if (device >= 0) GF_HIP(hipSetDevice(device));
@@ -145,6 +145,7 @@ struct PlainRun {
     std::vector<std::vector<uint8_t>> ext_at;
     DevBuf<GenRow<P>> d_rows;
     DevBuf<unsigned long long> d_live;
+    DevBuf<unsigned int> d_tickets;     /* one ticket counter per launch */
     DevBuf<float> freqA, freqB;
     DevBuf<gf_mutid> idsA, idsB;
     int64_t cap = 0;
@@ -170,7 +171,6 @@ struct PlainRun {
     int64_t tiles_cap = 0;              /* entries of d_cnt and of each d_src0 half */
     DevBuf<unsigned int> d_cnt;
     DevBuf<int> d_pre, d_src0, d_T;
-    DevBuf<unsigned int> d_tickets;     /* one ticket counter per launch */
     DevBuf<int> d_err;
     PinBuf<int> h_ring;                 /* slot count after each launch */
     std::vector<hipEvent_t> ev_ring;
@@ -620,7 +620,10 @@ struct PlainRun {
 
         int next_compact = start + interval;
         int t = 0;   /* generations done */
-        const int kpl = std::max(1, std::min(KMAX, generations_per_launch()));
+        /* one population compacts at every launch: shorter launches carry
+         * fewer lost slots (10 measured best for N = 1e5, theta = 2e5);
+         * several populations end launches at the compaction interval */
+        const int kpl = std::max(1, std::min(KMAX, generations_per_launch(P == 1 ? 10 : 16)));
         /* one population: a lost or fixed mutation never changes again, so
          * compacting every launch leaves every reported sample unchanged */
         const bool compact_every_launch = (P == 1 && interval > 0);
@@ -667,15 +670,16 @@ struct PlainRun {
         GF_HIP(hipStreamSynchronize(st));
 
         /* live counts */
-        std::vector<unsigned long long> lv((size_t)(G + 1) * 64);
-        GF_HIP(hipMemcpy(lv.data(), d_live.p, lv.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        DevBuf<unsigned long long> d_sum;
+        GF_TRY(d_sum.alloc((size_t)G + 1));
+        hipLaunchKernelGGL(k_sum_live, dim3((G + 1 + 255) / 256), dim3(256), 0, st, d_live.p, d_sum.p, G + 1);
+        GF_HIP(hipGetLastError());
+        std::vector<unsigned long long> lv((size_t)G + 1);
+        GF_HIP(hipMemcpyAsync(lv.data(), d_sum.p, lv.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
         R->live.assign(G + 1, 0);
         R->live[0] = live0;
-        for (int tt = 1; tt <= G; tt++) {
-            int64_t s = 0;
-            for (int k = 0; k < 64; k++) s += (int64_t)lv[(size_t)tt * 64 + k];
-            R->live[tt] = s;
-        }
+        for (int tt = 1; tt <= G; tt++) R->live[tt] = (int64_t)lv[(size_t)tt];
         float ims = 0;
         (void)hipEventElapsedTime(&ims, i0, i1);
         (void)hipEventDestroy(i0);
@@ -710,7 +714,7 @@ const char *gf_last_error(void) { return gf::get_error().c_str(); }
 int32_t gf_set_kernel_timing(int32_t enable) { gf::g_timing = enable != 0; return GF_OK; }
 
 int32_t gf_set_generations_per_launch(int32_t k) {
-    if (k < 1 || k > gf::KMAX) { gf::set_error("generations per launch must be in [1, 64]"); return GF_ERR_ARG; }
+    if (k < 0 || k > gf::KMAX) { gf::set_error("generations per launch must be in [1, 64] (0: default)"); return GF_ERR_ARG; }
     gf::g_kpl = k;
     return GF_OK;
 }

@@ -28,5 +28,6 @@ def test_error_path_without_device_call():
     h = capi.C.c_void_p()
     assert lib.gf_run(None, capi.C.byref(h)) == -1
     assert b"NULL" in lib.gf_last_error()
-    assert lib.gf_set_generations_per_launch(0) == -1
+    assert lib.gf_set_generations_per_launch(-1) == -1
     assert lib.gf_set_generations_per_launch(65) == -1
+    assert lib.gf_set_generations_per_launch(0) == 0    # back to the default

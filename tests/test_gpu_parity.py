@@ -41,7 +41,7 @@ def test_generations_per_launch_invariance(gpu, oracle, kpl):
     try:
         g = gpu.run(d)
     finally:
-        gpu.set_generations_per_launch(16)
+        gpu.set_generations_per_launch(0)
     assert_same(g, ref, "kpl=%d" % kpl)
 
 
