@@ -9,9 +9,9 @@
  *
  * Build: g++ -O2 -ffp-contract=off (never -ffast-math / -Ofast), see Makefile.
  * All float arithmetic is IEEE binary32 with round-to-nearest; every
- * transcendental is computed by the functions below from +,-,*,/ and sqrt
- * only, so the GPU library (which evaluates the same operation sequence)
- * must agree bit for bit.
+ * transcendental is computed by the functions below from +,-,*,/, sqrt and
+ * explicit fused multiply-add (fmaf) only, so the GPU library (which
+ * evaluates the same operation sequence) must agree bit for bit.
  *
  * Pinning (tests/test_oracle_*.py):
  *   philox4x32_10      Random123 known-answer vectors and vectors produced by
@@ -19,7 +19,8 @@
  *                      (oracle/Makefile -> oracle/_ref/philox_kat)
  *   u01                closed form (power-of-two scaling)
  *   exp_f, log_f       error bound vs long double (numpy longdouble)
- *   normcdfinv_f       error bound vs scipy.special.ndtri; symmetry
+ *   normcdfinv_f       error bound vs scipy.special.ndtri over u01's range;
+ *                      symmetry; monotonicity
  *   poiscdfinv/binomcdfinv  brute-force long-double CDF inversion with the
  *                      same 70-term truncation and mean thresholds
  *   drift update       closed forms (neutral -> identity selection, absorbing
@@ -39,7 +40,11 @@
  *     results do not depend on compaction or batching;
  *   - the new-mutation count uses Philox counter {0, g, pop, 1} instead of
  *     {1, g, pop, 0} (the latter collides with ID-keyed drift counters);
- *   - normcdfinv clamps u to at most 1 - 2^-24 (u == 1.0f would give +inf);
+ *   - the inverse normal CDF (CUDA's normcdfinvf in the reference, a closed
+ *     library) is a division-free polynomial approximation in the variable
+ *     w = -log2(4 s (1 - s)) of Giles (2011), coefficients fitted by
+ *     parallelpopgen_amd/csrc/fit_normcdfinv.py; u is clamped to at most
+ *     1 - 2^-24 (u == 1.0f would give +inf);
  *   - a population that is not drifted in a generation (not yet arisen,
  *     extinct) gets frequency 0 instead of NaN/stale memory;
  *   - the MSE integral uses a fixed-order two-level float scan
@@ -158,33 +163,43 @@ static float pow_f(float a, float b) {
     return exp_f(b * log_f(a));
 }
 
-/* Inverse standard normal CDF: Wichura, "Algorithm AS 241: The percentage
- * points of the normal distribution", Appl. Statist. 37 (1988) 477-484,
- * single-precision variant PPND7.  u is clamped to <= 1 - 2^-24. */
+/* Inverse standard normal CDF, p in [2^-33, 1] (u01's range), clamped to
+ * p <= 1 - 2^-24.  With s = min(p, 1 - p) and w = -log2(4 s (1 - s)) (the
+ * change of variable of Giles, "Approximating the erfinv function", GPU
+ * Computing Gems Jade Edition, 2011), ndtri(p) = (p - 1/2) R, R a polynomial
+ * in w - 4 for w < 8 and in sqrt(w) - 4.2031 beyond.  log2 of
+ * v = m 2^e (m in [sqrt(1/2), sqrt(2))) is e + f L(f), f = m - 1.  All
+ * polynomials by Horner's rule with fused multiply-adds. */
+static const float NCDF_LOG2[10] = {
+    0x1.715476p+0f, -0x1.71547p-1f, 0x1.ec70acp-2f, -0x1.715a6ep-2f, 0x1.2779eep-2f,
+    -0x1.eab71ep-3f, 0x1.a399dep-3f, -0x1.880824p-3f, 0x1.7a19cap-3f, -0x1.b74d82p-4f};
+static const float NCDF_CENTRAL[10] = {
+    0x1.1be4ccp+2f, 0x1.ea0ccp-2f, -0x1.c6ba7p-8f, -0x1.f793d2p-11f, 0x1.1d1adcp-13f,
+    -0x1.2aad7ap-18f, -0x1.b3f612p-21f, 0x1.bf18f8p-24f, 0x1.ff977ep-31f, -0x1.a7008p-31f};
+static const float NCDF_TAIL[9] = {
+    0x1.2dd49ap+3f, 0x1.2fa2d4p+1f, 0x1.a6a788p-8f, -0x1.b5be06p-9f, 0x1.76b48ep-10f,
+    -0x1.b1259cp-11f, 0x1.fbb8cap-12f, -0x1.3a9bp-13f, 0x1.6a392ep-17f};
+
+static float poly_fma(const float *c, int n, float x) {
+    float acc = c[n - 1];
+    for (int k = n - 2; k >= 0; k--) acc = fmaf(acc, x, c[k]);
+    return acc;
+}
+
 static float normcdfinv_f(float p) {
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
-    float q = p - 0.5f;
-    if (fabsf(q) <= 0.425f) {
-        float r = 0.180625f - q * q;
-        float num = ((5.9109374720e+01f * r + 1.5929113202e+02f) * r + 5.0434271938e+01f) * r + 3.3871327179e+00f;
-        float den = ((6.7187563600e+01f * r + 7.8757757664e+01f) * r + 1.7895169469e+01f) * r + 1.0f;
-        return q * num / den;
-    }
-    float r = q < 0.0f ? p : 1.0f - p;
-    r = sqrtf(-log_f(r));
-    float z;
-    if (r <= 5.0f) {
-        r = r - 1.6f;
-        float num = ((1.7023821103e-01f * r + 1.3067284816e+00f) * r + 2.7568153900e+00f) * r + 1.4234372777e+00f;
-        float den = (1.2021132975e-01f * r + 7.3700164250e-01f) * r + 1.0f;
-        z = num / den;
-    } else {
-        r = r - 5.0f;
-        float num = ((1.7337203997e-02f * r + 4.2868294337e-01f) * r + 3.0812263860e+00f) * r + 6.6579046435e+00f;
-        float den = (1.2258202635e-02f * r + 2.4197894225e-01f) * r + 1.0f;
-        z = num / den;
-    }
-    return q < 0.0f ? -z : z;
+    const float q = p - 0.5f;
+    const float s = q < 0.0f ? p : 1.0f - p;
+    const float v = (4.0f * s) * (1.0f - s);
+    const uint32_t b = f2bits(v);
+    int e = (int)(b >> 23) - 127;
+    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = 0.5f * m; e += 1; }
+    const float f = m - 1.0f;
+    const float w = -(float)e - f * poly_fma(NCDF_LOG2, 10, f);
+    const float r = w < 8.0f ? poly_fma(NCDF_CENTRAL, 10, w - 4.0f)
+                             : poly_fma(NCDF_TAIL, 9, sqrtf(w) - 0x1.0cff86p+2f);
+    return q * r;
 }
 
 /* round half away from zero (CUDA round(), used at shared.cuh:255,266) */

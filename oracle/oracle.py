@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_FMA_PATH = os.path.join(HERE, "liboracle_fma.so")
 
 _lib = None
 
@@ -19,13 +20,26 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE, "all"])
 
 
+def _cpu_has_fma():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return " fma " in line + " "
+    except OSError:
+        pass
+    return False
+
+
 def load():
+    """Opens the oracle library: the -mfma build where the CPU has FMA (same
+    results, fmaf inline), else the portable one."""
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    if not os.path.exists(LIB_PATH) or not os.path.exists(LIB_FMA_PATH):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(LIB_FMA_PATH if _cpu_has_fma() else LIB_PATH)
     vp = C.c_void_p
     i32, i64, f32 = C.c_int32, C.c_int64, C.c_float
     sig = {

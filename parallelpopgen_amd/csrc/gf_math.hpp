@@ -126,26 +126,6 @@ GF_HD float log_f(float x) {
     return fe * 0.693359375f + (lm + fe * -2.12194440e-4f);
 }
 
-/* log_f without its special cases: the same operations, so the same value,
- * for finite normal x > 0 */
-GF_HD float log_f_mid(float x) {
-    GF_STRICT_FP
-    const uint32_t b = f2bits(x);
-    int e = (int)((b >> 23) & 0xffu) - 127;
-    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
-    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
-    float f = m - 1.0f;
-    float s = f / (2.0f + f);
-    float s2 = s * s;
-    float q = 0.111111111f;
-    q = q * s2 + 0.142857143f;
-    q = q * s2 + 0.2f;
-    q = q * s2 + 0.333333333f;
-    float lm = (s + s) + (s + s) * (s2 * q);
-    float fe = (float)e;
-    return fe * 0.693359375f + (lm + fe * -2.12194440e-4f);
-}
-
 GF_HD float pow_f(float a, float b) {
     GF_STRICT_FP
     if (a == 1.0f || b == 0.0f) return 1.0f;
@@ -153,40 +133,51 @@ GF_HD float pow_f(float a, float b) {
     return exp_f(b * log_f(a));
 }
 
-/* inverse standard normal CDF, Wichura AS 241 PPND7 (Appl. Statist. 37,
- * 1988, 477-484); u clamped to <= 1 - 2^-24 so the result stays finite */
+/* Inverse standard normal CDF for p in [2^-33, 1] (the range of u01; p is
+ * clamped to <= 1 - 2^-24 so the result stays finite).  Method: the change
+ * of variable of Giles ("Approximating the erfinv function", GPU Computing
+ * Gems Jade Edition, 2011): with s = min(p, 1 - p) and w = -log2(4 s (1 - s)),
+ * ndtri(p) / (p - 1/2) is a smooth function of w, given by one polynomial
+ * for w < 8 (p within about 9.8e-4 of 0 or 1 is the tail) and one in
+ * sqrt(w) beyond.  No division; the tail branch runs in few waves.
+ * Coefficients fitted by parallelpopgen_amd/csrc/fit_normcdfinv.py; error
+ * against scipy.special.ndtri below 3.4e-7 relative (|z| > 1e-3) and
+ * 1.6e-6 absolute over every u01 value tested. */
+static constexpr float kNcdfLog2[10] = {
+    0x1.715476p+0f, -0x1.71547p-1f, 0x1.ec70acp-2f, -0x1.715a6ep-2f, 0x1.2779eep-2f,
+    -0x1.eab71ep-3f, 0x1.a399dep-3f, -0x1.880824p-3f, 0x1.7a19cap-3f, -0x1.b74d82p-4f};
+static constexpr float kNcdfCentral[10] = {
+    0x1.1be4ccp+2f, 0x1.ea0ccp-2f, -0x1.c6ba7p-8f, -0x1.f793d2p-11f, 0x1.1d1adcp-13f,
+    -0x1.2aad7ap-18f, -0x1.b3f612p-21f, 0x1.bf18f8p-24f, 0x1.ff977ep-31f, -0x1.a7008p-31f};
+static constexpr float kNcdfTail[9] = {
+    0x1.2dd49ap+3f, 0x1.2fa2d4p+1f, 0x1.a6a788p-8f, -0x1.b5be06p-9f, 0x1.76b48ep-10f,
+    -0x1.b1259cp-11f, 0x1.fbb8cap-12f, -0x1.3a9bp-13f, 0x1.6a392ep-17f};
+
+/* c[0] + x (c[1] + x (c[2] + ...)), fused multiply-adds */
+template <int N>
+GF_HD float horner_fma(const float (&c)[N], float x) {
+    float acc = c[N - 1];
+#pragma unroll
+    for (int k = N - 2; k >= 0; k--) acc = __builtin_fmaf(acc, x, c[k]);
+    return acc;
+}
+
 GF_HD float normcdfinv_f(float p) {
     GF_STRICT_FP
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
-    const bool central = __builtin_fabsf(q) <= 0.425f;
-    /* one rational evaluation with per-lane coefficients; the tail's
-     * sqrt(-log(.)) only runs for lanes outside the central region */
+    const float s = q < 0.0f ? p : 1.0f - p;        /* (0, 1/2], exact */
+    const float v = (4.0f * s) * (1.0f - s);        /* (0, 1], normal */
+    const uint32_t b = f2bits(v);
+    int e = (int)(b >> 23) - 127;
+    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = 0.5f * m; e += 1; }
+    const float f = m - 1.0f;
+    const float w = -(float)e - f * horner_fma(kNcdfLog2, f);   /* -log2(v) >= 0 */
     float r;
-    float a0, a1, a2, a3, b1, b2, b3;
-    if (central) {
-        r = 0.180625f - q * q;
-        a3 = 5.9109374720e+01f; a2 = 1.5929113202e+02f; a1 = 5.0434271938e+01f; a0 = 3.3871327179e+00f;
-        b3 = 6.7187563600e+01f; b2 = 7.8757757664e+01f; b1 = 1.7895169469e+01f;
-    } else {
-        float t = q < 0.0f ? p : 1.0f - p;
-        t = __builtin_sqrtf(-log_f_mid(t));   /* 2^-33 <= t < 0.075: normal, positive, finite */
-        b3 = 0.0f;
-        if (t <= 5.0f) {
-            r = t - 1.6f;
-            a3 = 1.7023821103e-01f; a2 = 1.3067284816e+00f; a1 = 2.7568153900e+00f; a0 = 1.4234372777e+00f;
-            b2 = 1.2021132975e-01f; b1 = 7.3700164250e-01f;
-        } else {
-            r = t - 5.0f;
-            a3 = 1.7337203997e-02f; a2 = 4.2868294337e-01f; a1 = 3.0812263860e+00f; a0 = 6.6579046435e+00f;
-            b2 = 1.2258202635e-02f; b1 = 2.4197894225e-01f;
-        }
-    }
-    const float num = ((a3 * r + a2) * r + a1) * r + a0;
-    /* b3 = 0 in the tails: 0*r + b2 == b2 exactly */
-    const float den = ((b3 * r + b2) * r + b1) * r + 1.0f;
-    const float z = ((central ? q : 1.0f) * num) / den;
-    return (central || q >= 0.0f) ? z : -z;
+    if (w < 8.0f) r = horner_fma(kNcdfCentral, w - 4.0f);
+    else r = horner_fma(kNcdfTail, __builtin_sqrtf(w) - 0x1.0cff86p+2f);
+    return q * r;
 }
 
 GF_HD float round_away(float v) {
