@@ -112,23 +112,20 @@ static inline float scale2(float p, int n) {
 }
 
 /* exp: Cody-Waite reduction x = n ln2 + r, |r| <= ln2/2, degree-7 Taylor
- * polynomial in Horner form, exact power-of-two scaling. */
+ * polynomial in Horner form (reduction and Horner steps fused
+ * multiply-adds), exact power-of-two scaling. */
 static float exp_f(float x) {
     if (x != x) return x;
     if (x > 88.72283935546875f) return INFINITY;
     if (x < -103.97208404541015625f) return 0.0f;
     float t = x * 1.44269502162933349609375f;      /* log2(e) rounded to float */
     float n = floorf(t + 0.5f);
-    float r = x - n * 0.693359375f;                 /* ln2 high part, 9 bits: exact */
-    r = r - n * -2.12194440e-4f;                    /* ln2 - 0.693359375 */
-    float p = 1.98412698e-4f;                       /* 1/5040 */
-    p = p * r + 1.38888889e-3f;                     /* 1/720 */
-    p = p * r + 8.33333333e-3f;                     /* 1/120 */
-    p = p * r + 4.16666667e-2f;                     /* 1/24  */
-    p = p * r + 1.66666667e-1f;                     /* 1/6   */
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    float r = fmaf(-n, 0.693359375f, x);            /* x - n ln2_hi (fused) */
+    r = fmaf(n, 2.12194440e-4f, r);                 /* - n (ln2 - ln2_hi) */
+    const float c[8] = {1.0f, 1.0f, 0.5f, 1.66666667e-1f, 4.16666667e-2f,   /* 1/k! */
+                        8.33333333e-3f, 1.38888889e-3f, 1.98412698e-4f};
+    float p = c[7];
+    for (int k = 6; k >= 0; k--) p = fmaf(p, r, c[k]);
     return scale2(p, (int)n);
 }
 
@@ -360,8 +357,11 @@ static inline void drift_one(const GenRow &t, float *x, const gf_mutid &id, int 
         float c1 = (F + h) - (h * F);
         float c2 = (F + (2.0f * h)) - ((2.0f * h) * F);
         float om = 1.0f - im;
-        float num = (((s * im) * im) + im) + (((c1 * s) * im) * om);
-        float den = (((im * im) * s) + (((c2 * s) * im) * om)) + 1.0f;
+        /* Eq. 13 as x + s x (x + c1 (1 - x)) over 1 + s x (x + c2 (1 - x)),
+         * with fused multiply-adds */
+        float sx = s * im;
+        float num = fmaf(sx, fmaf(c1, om, im), im);
+        float den = fmaf(sx, fmaf(c2, om, im), 1.0f);
         float xs = num / den;
         float N = (float)Nc;
         float mean = xs * N;

@@ -156,9 +156,11 @@ __device__ __forceinline__ void generation_update(float *x, const Row &R, int g,
         } else {
             float s = sel(p, g, im, R);
             s = s > -1.0f ? s : -1.0f;
+            /* Eq. 13: (x + s x (x + c1 (1-x))) / (1 + s x (x + c2 (1-x))) */
             const float om = 1.0f - im;
-            const float num = (((s * im) * im) + im) + (((R.c1[p] * s) * im) * om);
-            const float den = (((im * im) * s) + (((R.c2[p] * s) * im) * om)) + 1.0f;
+            const float sx = s * im;
+            const float num = __builtin_fmaf(sx, __builtin_fmaf(R.c1[p], om, im), im);
+            const float den = __builtin_fmaf(sx, __builtin_fmaf(R.c2[p], om, im), 1.0f);
             xs = num / den;
         }
         const float N = R.fN[p];

@@ -80,24 +80,31 @@ GF_HD float scale2(float p, int n) {
 }
 
 /* exp(x): Cody-Waite split of ln 2 (0.693359375 + -2.12194440e-4), degree-7
- * Taylor polynomial on |r| <= ln2/2, exact scaling by 2^n. */
+ * Taylor polynomial on |r| <= ln2/2 by Horner's rule, each step and the
+ * reduction one fused multiply-add; exact scaling by 2^n. */
+GF_HD float exp_poly(float x, float &n) {
+    GF_STRICT_FP
+    const float t = x * 1.44269502162933349609375f;
+    n = __builtin_floorf(t + 0.5f);
+    float r = __builtin_fmaf(-n, 0.693359375f, x);
+    r = __builtin_fmaf(n, 2.12194440e-4f, r);
+    float p = 1.98412698e-4f;
+    p = __builtin_fmaf(p, r, 1.38888889e-3f);
+    p = __builtin_fmaf(p, r, 8.33333333e-3f);
+    p = __builtin_fmaf(p, r, 4.16666667e-2f);
+    p = __builtin_fmaf(p, r, 1.66666667e-1f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return __builtin_fmaf(p, r, 1.0f);
+}
+
 GF_HD float exp_f(float x) {
     GF_STRICT_FP
     if (x != x) return x;
     if (x > 88.72283935546875f) return __builtin_inff();
     if (x < -103.97208404541015625f) return 0.0f;
-    float t = x * 1.44269502162933349609375f;
-    float n = __builtin_floorf(t + 0.5f);
-    float r = x - n * 0.693359375f;
-    r = r - n * -2.12194440e-4f;
-    float p = 1.98412698e-4f;
-    p = p * r + 1.38888889e-3f;
-    p = p * r + 8.33333333e-3f;
-    p = p * r + 4.16666667e-2f;
-    p = p * r + 1.66666667e-1f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    float n;
+    const float p = exp_poly(x, n);
     return scale2(p, (int)n);
 }
 
@@ -269,18 +276,8 @@ GF_HD int exit_index(float mean) {
  * for -87 <= x <= 88 (no NaN, no overflow, the scale 2^n stays normal) */
 GF_HD float exp_f_mid(float x) {
     GF_STRICT_FP
-    float t = x * 1.44269502162933349609375f;
-    float n = __builtin_floorf(t + 0.5f);
-    float r = x - n * 0.693359375f;
-    r = r - n * -2.12194440e-4f;
-    float p = 1.98412698e-4f;
-    p = p * r + 1.38888889e-3f;
-    p = p * r + 8.33333333e-3f;
-    p = p * r + 4.16666667e-2f;
-    p = p * r + 1.66666667e-1f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    float n;
+    const float p = exp_poly(x, n);
     return p * pow2i((int)n);
 }
 
@@ -334,6 +331,24 @@ static __host__ __device__ __attribute__((noinline)) int binomcdfinv(float r, fl
     return 70;
 }
 
+/* Correctly rounded square root of a positive normal float x >= 2^-96 (the
+ * variance of the normal approximation, at least 36/N there).  On the
+ * device: the hardware estimate (within one ulp) moved by one ulp where the
+ * fused residual x - y*y says so, which is the compiler's own correctly
+ * rounded expansion without its scaling of tiny inputs and its zero,
+ * infinity and NaN cases; the host's sqrtf is correctly rounded. */
+GF_HD float sqrt_pos(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y = __builtin_amdgcn_sqrtf(x);
+    const float ydn = bits2f(f2bits(y) - 1u), yup = bits2f(f2bits(y) + 1u);
+    float r = __builtin_fmaf(-ydn, y, x) <= 0.0f ? ydn : y;
+    r = __builtin_fmaf(-yup, y, x) > 0.0f ? yup : r;
+    return r;
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+
 /* approximate binomial draw (reference shared.cu:10-18): the two Poisson
  * (or binomial, N < 100) tails share one CDF inversion; the rest is the
  * rounded normal approximation */
@@ -347,7 +362,7 @@ GF_HD int binom_draw(uint32_t bits, float mean, float var, float N) {
         const int k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv(u, m);
         return lo ? k : f2i_sat(N - (float)k);
     }
-    return f2i_sat(round_away(normcdfinv_f(u) * __builtin_sqrtf(var) + mean));
+    return f2i_sat(round_away(normcdfinv_f(u) * sqrt_pos(var) + mean));
 }
 
 /* approximate Poisson draw (reference shared.cuh:251-256) */
@@ -360,7 +375,7 @@ GF_HD int pois_draw(uint32_t bits, float mean, float var, float N) {
         const int k = poiscdfinv(u, lo ? mean : N - mean);
         return lo ? k : f2i_sat(N - (float)k);
     }
-    return f2i_sat(round_away(normcdfinv_f(u) * __builtin_sqrtf(var) + mean));
+    return f2i_sat(round_away(normcdfinv_f(u) * sqrt_pos(var) + mean));
 }
 
 /* ----- model helpers ----- */
