@@ -9,7 +9,8 @@
  *   (reference binom, spectrum.cu:111-139).  Each term is truncated to 2^-32
  *   fixed point and summed in 64-bit integers (LDS, then global), so the
  *   result is independent of summation order.  Terms with exponent < -23
- *   truncate to 0 and are skipped.
+ *   truncate to 0 and are skipped; each lane visits only the levels around
+ *   its mode where they can occur.
  */
 #ifndef GF_SPECTRUM_HPP_
 #define GF_SPECTRUM_HPP_
@@ -67,15 +68,36 @@ __global__ __launch_bounds__(WG) void k_sfs(const float *__restrict__ freq, cons
         if (pf > 0.0f && pf < 1.0f) {
             const float lp = log_f(pf);
             const float lq = log_f(1.0f - pf);
-            for (int k = 0; k <= n; k++) {
+            /* The terms with t >= -23 (the only ones that survive the 2^-32
+             * truncation) form one run of levels around the mode: t is a
+             * concave function of k (log C(n,k) is concave, the rest linear)
+             * evaluated with rounding errors far below 1, so walking out
+             * from the mode until t < -24 on each side visits every one of
+             * them, and adds exactly the terms a loop over all n + 1 levels
+             * would. */
+            auto level = [&](int k) {
                 const float coeff = k < half_n ? lc[k] : lc[n - k];
-                const float t = (coeff + (float)k * lp) + (float)(n - k) * lq;
-                if (t < -23.0f) continue;
+                return (coeff + (float)k * lp) + (float)(n - k) * lq;
+            };
+            auto add = [&](int k, float t) {
+                if (t < -23.0f) return;
                 const unsigned long long v = (unsigned long long)(exp_f(t) * 4294967296.0f);
                 if (v) {
                     if (LDS) atomicAdd(&sh[k], v);
                     else atomicAdd(&out[k], v);
                 }
+            };
+            int km = (int)((float)(n + 1) * pf);
+            km = km > n ? n : km;
+            for (int k = km; k <= n; k++) {
+                const float t = level(k);
+                if (t < -24.0f) break;
+                add(k, t);
+            }
+            for (int k = km - 1; k >= 0; k--) {
+                const float t = level(k);
+                if (t < -24.0f) break;
+                add(k, t);
             }
         } else {
             if (LDS) atomicAdd(&sh[n + 1], 1ull);

@@ -156,6 +156,25 @@ def test_spectrum_matches_oracle(gpu, oracle):
         assert np.array_equal(s_g.view(np.uint32), s_o.view(np.uint32)), n
 
 
+def test_sfs_all_frequencies_matches_oracle(gpu, oracle):
+    """sample SFS of rows covering the whole frequency range (uniform, near
+    0 and 1, on the 1/N grid, at level boundaries (n+1)p integral, and 0 / 1
+    themselves): each lane visits only the levels around its mode, and the
+    result must equal the oracle's loop over every level bit for bit"""
+    rng = np.random.default_rng(11)
+    N = 20000
+    rows = [rng.uniform(0, 1, 10000),
+            np.exp(rng.uniform(np.log(1e-7), 0, 10000)),
+            1 - np.exp(rng.uniform(np.log(1e-7), 0, 10000)),
+            rng.integers(0, N + 1, 10000) / N]
+    for n in [2, 7, 100, 1000, 5001]:
+        grid = (np.arange(1, n + 1) / (n + 1)).astype(np.float32)
+        row = np.concatenate(rows + [grid, np.array([0.0, 1.0, 0.5])]).astype(np.float32)
+        s_g = gpu.site_frequency_spectrum(row, n, 1e9)
+        s_o = oracle.site_frequency_spectrum(row, n, 1e9)
+        assert np.array_equal(s_g.view(np.uint32), s_o.view(np.uint32)), n
+
+
 def test_final_sfs_on_device_matches_host_path(gpu, oracle):
     """gf_result_site_frequency_spectrum bins the device-resident final
     state: identical to the spectrum of the copied-out sample (and to the
