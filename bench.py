@@ -12,7 +12,7 @@ live (segregating or preserved) mutations after each generation's update,
 divided by the wall time of the step.
 
 Prints ONE JSON line (rank 0, N = 1).  Also reports:
-  roofline      the drift kernel (k_advance) against MI355X HBM peak, with
+  roofline      the drift kernel (k_advance_fused) against MI355X HBM peak, with
                 8 algorithmic bytes per mutation-generation (4 B read + 4 B
                 written state), timed with HIP events on the library stream;
                 traffic = measured HBM bytes per launch from rocprofv3
@@ -22,6 +22,10 @@ Prints ONE JSON line (rank 0, N = 1).  Also reports:
                 generations, on the host's cores
 
 Other workloads (--workload), each printing one JSON line of its own:
+  three_population
+             the three-population scenario (growth, bottleneck, inbreeding,
+             migration, sine-wave selection, preserve every 100), 2,000
+             generations + SFS n = 50 per population; metric as above
   batched    a parameter sweep of 4,096 single-population simulations
              (scenarios.parameter_sweep) in one gf_batch_run, against the
              same simulations run back to back through gf_run (a stratified
@@ -51,7 +55,8 @@ def parse():
     ap.add_argument("--generations", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-generations", type=int, default=0, help="generations of the CPU sample (0: auto)")
-    ap.add_argument("--workload", choices=["plain_large", "batched", "surrogate"], default="plain_large")
+    ap.add_argument("--workload", choices=["plain_large", "three_population", "batched", "surrogate"],
+                    default="plain_large")
     ap.add_argument("--sims", type=int, default=4096, help="batched: simulations in the sweep")
     ap.add_argument("--plain-sample", type=int, default=32, help="batched: simulations timed back to back")
     ap.add_argument("--train-draws", type=int, default=1000000, help="surrogate: training draws")
@@ -196,6 +201,51 @@ def bench_surrogate(a):
     }
 
 
+def bench_three_population(a, capi, scenarios):
+    """Three populations (exponential growth 2e4 -> 2e5, a bottleneck, F = 0.1,
+    migration 1e-4, sine-wave selection, preserve every 100 generations),
+    2,000 generations, final SFS n = 50 of each population on the device."""
+    G = 2000
+    capi.set_kernel_timing(True)
+    d = scenarios.three_population(G=G)
+    pairs = [(0, 50), (1, 50), (2, 50)]
+    for _ in range(a.warmup):
+        capi.run(d, final_sfs=pairs)
+    total_t, total_mg, adv_ms, adv_launch = 0.0, 0, 0.0, 0
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        r = capi.run(d, timing=True, final_sfs=pairs)
+        total_t += time.perf_counter() - t0
+        total_mg += int(r["live"][1:].sum())
+        adv_ms += r["timing"]["advance_ms"]
+        adv_launch += r["timing"]["advance_launches"]
+    return {
+        "metric": "mutation-generations/s",
+        "value": total_mg / total_t,
+        "unit": "mutation-generations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * total_t / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (simulator's own mutation-selection-equilibrium start)",
+        "config": {
+            "workload": "three_population: P=3, N0=2e4 (population 0 grows exponentially to 2e5 over "
+                        "generations 0-500, population 1 bottleneck 2e3 in [500,600), population 2 F=0.1), "
+                        "migration 1e-4, mu=1e-9, L=2e8, h=0.3, sine-wave selection (gamma amplitude 5, "
+                        "period 200), preserve every 100, 2000 generations, + SFS n=50 per population; "
+                        "a mutation counts once per generation (not once per population)",
+            "mutation_generations_per_step": total_mg // a.steps,
+            "advance_launches_per_step": adv_launch // a.steps,
+            "advance_avg_launch_ms": adv_ms / max(adv_launch, 1),
+            "parallelism": "single GPU",
+        },
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -210,6 +260,9 @@ def main():
         return
     if a.workload == "surrogate":
         print(json.dumps(bench_surrogate(a)))
+        return
+    if a.workload == "three_population":
+        print(json.dumps(bench_three_population(a, capi, scenarios)))
         return
     capi.set_kernel_timing(True)
     for _ in range(a.warmup):
