@@ -88,16 +88,20 @@ typedef struct gf_sim_desc {
 typedef struct gf_result gf_result;
 
 /* Kernel timing collected while gf_run executes (only when enabled with
- * gf_set_kernel_timing(1)); durations in milliseconds from HIP events
- * recorded on the library's stream around each launch. */
+ * gf_set_kernel_timing); durations in milliseconds from HIP events recorded
+ * on the library's stream around the timed launches (every launch, or every
+ * s-th); the average duration of a launch is advance_ms /
+ * advance_timed_launches. */
 typedef struct gf_timing {
-    double  advance_ms;        /* migration_selection_drift + add_new_mutations */
-    int64_t advance_launches;
-    double  compact_ms;        /* flag + scan + scatter */
+    double  advance_ms;        /* migration_selection_drift + add_new_mutations, timed launches */
+    int64_t advance_launches;  /* all launches */
+    double  compact_ms;        /* flag + scan + scatter, timed launches */
     int64_t compact_launches;
     double  init_ms;           /* mutation-selection-equilibrium initialisation */
     double  total_ms;          /* whole gf_run, host clock */
     int64_t advance_slot_generations;  /* slot x generation updates issued by the advance kernel */
+    int64_t advance_timed_launches;
+    int64_t compact_timed_launches;
 } gf_timing;
 
 /* Runs one simulation (replaces GO_Fish::run_sim, go_fish_impl.cuh:553-730,
@@ -288,6 +292,8 @@ int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void 
 
 /* ----- runtime ----- */
 const char *gf_last_error(void);
+/* 0: off; 1: every kernel launch is timed; s > 1: every s-th launch (a
+ * pair of timing events costs a few microseconds of the stream's time) */
 int32_t gf_set_kernel_timing(int32_t enable);
 /* Number of generations the drift kernel advances per launch at most
  * (temporal blocking depth); 1 gives one generation per launch, 0 restores

@@ -50,7 +50,8 @@ class gf_timing(C.Structure):
     _fields_ = [("advance_ms", C.c_double), ("advance_launches", C.c_int64),
                 ("compact_ms", C.c_double), ("compact_launches", C.c_int64),
                 ("init_ms", C.c_double), ("total_ms", C.c_double),
-                ("advance_slot_generations", C.c_int64)]
+                ("advance_slot_generations", C.c_int64),
+                ("advance_timed_launches", C.c_int64), ("compact_timed_launches", C.c_int64)]
 
 
 class gf_batch_sim(C.Structure):
@@ -296,7 +297,8 @@ def site_frequency_spectrum(freq_row, sample_size, num_sites, device=-1):
 
 
 def set_kernel_timing(on):
-    check(load().gf_set_kernel_timing(1 if on else 0))
+    """on: False / 0 off, True / 1 every drift launch timed, s > 1 every s-th"""
+    check(load().gf_set_kernel_timing(int(on)))
 
 
 def set_generations_per_launch(k):

@@ -246,7 +246,7 @@ struct BatchRun {
         R->sfs.assign((size_t)S * n, 0.0f);
         R->num_mutations.assign(S, 0);
         if (keep) { R->freq.resize(S); R->ids.resize(S); }
-        t_adv.on = t_cmp.on = timing_enabled();
+        t_adv.stride = t_cmp.stride = timing_stride();
         auto t0 = std::chrono::steady_clock::now();
         GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         hipEvent_t i0, i1;
@@ -454,8 +454,10 @@ struct BatchRun {
         R->timing.init_ms = ims;
         R->timing.advance_ms = t_adv.total_ms();
         R->timing.advance_launches = t_adv.count();
+        R->timing.advance_timed_launches = t_adv.timed();
         R->timing.compact_ms = t_cmp.total_ms();
         R->timing.compact_launches = t_cmp.count();
+        R->timing.compact_timed_launches = t_cmp.timed();
         R->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return GF_OK;
     }

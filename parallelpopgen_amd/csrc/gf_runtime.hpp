@@ -20,7 +20,7 @@ namespace gf {
 
 void set_error(const std::string &msg);
 const std::string &get_error();
-bool timing_enabled();
+int timing_stride();   /* 0: kernel timing off; s: every s-th launch timed */
 int generations_per_launch(int dflt);
 
 struct Status {
@@ -131,19 +131,26 @@ inline int readback(void *dst, const void *src, size_t bytes, hipStream_t st) {
     return GF_OK;
 }
 
-/* HIP-event based kernel timer (on the library's stream) */
+/* HIP-event based kernel timer (on the library's stream): with stride s,
+ * launches 0, s, 2s, ... are bracketed by a pair of timing events (each
+ * pair adds a few microseconds to the stream's timeline); every launch is
+ * counted. */
 struct KernelTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    bool on = false;
+    int stride = 0;            /* 0: off */
+    int64_t launches = 0;
     hipStream_t stream = nullptr;
     void begin(hipStream_t s, hipEvent_t &a) {
-        if (!on) return;
+        a = nullptr;
+        const bool timed = stride > 0 && launches % stride == 0;
+        launches++;
+        if (!timed) return;
         (void)hipEventCreate(&a);
         (void)hipEventRecord(a, s);
         stream = s;
     }
     void end(hipEvent_t a) {
-        if (!on) return;
+        if (!a) return;
         hipEvent_t b;
         (void)hipEventCreate(&b);
         (void)hipEventRecord(b, stream);
@@ -158,7 +165,8 @@ struct KernelTimer {
         }
         return t;
     }
-    int64_t count() const { return (int64_t)ev.size(); }
+    int64_t count() const { return launches; }
+    int64_t timed() const { return (int64_t)ev.size(); }
     ~KernelTimer() {
         for (auto &e : ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     }

@@ -51,8 +51,8 @@ namespace gf {
 static thread_local std::string t_err;
 void set_error(const std::string &m) { t_err = m; }
 const std::string &get_error() { return t_err; }
-static bool g_timing = false;
-bool timing_enabled() { return g_timing; }
+static int g_timing = 0;
+int timing_stride() { return g_timing; }
 static int g_kpl = 0;    /* generations per advance launch (1..KMAX); 0: the caller's default */
 int generations_per_launch(int dflt) { return g_kpl > 0 ? g_kpl : dflt; }
 
@@ -574,7 +574,7 @@ struct PlainRun {
         k1 = (uint32_t)d->seed2;
         R->P = P;
         R->G = G;
-        t_adv.on = t_cmp.on = timing_enabled();
+        t_adv.stride = t_cmp.stride = timing_stride();
         auto t_start = std::chrono::steady_clock::now();
         GF_TRY(select_device(d->device));
         GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -687,8 +687,10 @@ struct PlainRun {
         R->timing.init_ms = ims;
         R->timing.advance_ms = t_adv.total_ms();
         R->timing.advance_launches = t_adv.count();
+        R->timing.advance_timed_launches = t_adv.timed();
         R->timing.compact_ms = t_cmp.total_ms();
         R->timing.compact_launches = t_cmp.count();
+        R->timing.compact_timed_launches = t_cmp.timed();
         R->timing.advance_slot_generations = slot_gens;
         R->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         return GF_OK;
@@ -711,7 +713,11 @@ extern "C" {
 
 const char *gf_last_error(void) { return gf::get_error().c_str(); }
 
-int32_t gf_set_kernel_timing(int32_t enable) { gf::g_timing = enable != 0; return GF_OK; }
+int32_t gf_set_kernel_timing(int32_t enable) {
+    if (enable < 0) { gf::set_error("kernel timing: enable must be >= 0"); return GF_ERR_ARG; }
+    gf::g_timing = enable;
+    return GF_OK;
+}
 
 int32_t gf_set_generations_per_launch(int32_t k) {
     if (k < 0 || k > gf::KMAX) { gf::set_error("generations per launch must be in [1, 64] (0: default)"); return GF_ERR_ARG; }
