@@ -1,0 +1,73 @@
+"""Edge cases of the GPU run_sim path against the oracle: a run in which no
+mutation ever arises, the maximum of 8 populations, a time sample in every
+generation, and the reference's parameter checks (check_sim_parameters,
+go_fish_impl.cuh:431-452) rejecting a run before anything is simulated."""
+import numpy as np
+import pytest
+
+from parallelpopgen_amd import scenarios as S
+from parallelpopgen_amd.capi import GF_INIT_BLANK, GoFishError
+from test_gpu_parity import assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+def test_no_mutations_at_all(gpu, oracle):
+    """blank start and mu = 0: every sample is empty, every live count 0,
+    and the SFS of an empty sample puts all sites in bin 0"""
+    d = S.constant(N=1000, L=1e6, G=50, mu=0.0, init_mode=GF_INIT_BLANK, sample_every=10)
+    g = gpu.run(d)
+    o = oracle.run(d)
+    assert_same(g, o, "empty")
+    assert all(s["num_mutations"] == 0 for s in g["samples"])
+    assert not g["live"].any() and len(g["ids"]) == 0
+    empty = np.zeros(0, np.float32)
+    s_g = gpu.site_frequency_spectrum(empty, 10, d.num_sites)
+    s_o = oracle.site_frequency_spectrum(empty, 10, d.num_sites)
+    assert np.array_equal(s_g.view(np.uint32), s_o.view(np.uint32))
+    assert s_g[0] == np.float32(d.num_sites) and not s_g[1:].any()
+
+
+def test_eight_populations(gpu, oracle):
+    """P = 8 (the maximum): migration between all pairs, selection, one
+    population inbred"""
+    d = S.constant(N=1500, L=3e7, G=80, P=8, m=2e-3, s=S.selection_from_gamma(-5.0, 1500, 0.0), h=0.3,
+                   sample_every=40)
+    d.params["F"][:, 3] = 0.25
+    g = gpu.run(d)
+    o = oracle.run(d)
+    assert_same(g, o, "P=8")
+    assert g["samples"][-1]["num_mutations"] > 1000
+
+
+def test_sample_every_generation(gpu, oracle):
+    d = S.constant(N=800, L=2e7, G=40, s=S.selection_from_gamma(3.0, 800, 0.0), sample_every=1)
+    g = gpu.run(d)
+    o = oracle.run(d)
+    assert len(g["samples"]) == 41
+    assert_same(g, o, "every generation")
+
+
+@pytest.mark.parametrize("field,value,text", [
+    ("F", 1.5, "inbreeding coefficient > 1"),
+    ("F", -0.1, "inbreeding coefficient < 0"),
+    ("mu", -1e-9, "mu_rate < 0"),
+])
+def test_parameter_errors(gpu, oracle, field, value, text):
+    """a bad parameter at generation 7 fails the whole run with the
+    reference's message, on the GPU and in the oracle"""
+    d = S.constant(N=1000, L=1e6, G=20)
+    d.params[field][7, 0] = value
+    with pytest.raises(GoFishError) as e:
+        gpu.run(d)
+    assert text in str(e.value) and "generation 7" in str(e.value)
+    with pytest.raises(RuntimeError):
+        oracle.run(d)
+
+
+def test_migration_must_sum_to_one(gpu):
+    d = S.constant(N=1000, L=1e6, G=10, P=2, m=0.01)
+    d.migration[3, 0, 0] = 0.5      # column 0 of generation 3 now sums to 0.51
+    with pytest.raises(GoFishError) as e:
+        gpu.run(d)
+    assert "migration rate does not sum to 1" in str(e.value)
