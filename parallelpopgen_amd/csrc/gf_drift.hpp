@@ -349,7 +349,7 @@ struct FusedLaunch {
     const int *src0_in;           /* fragmented input: first input tile of each tile of this launch */
     const int *T_in;              /* fragmented input: number of input tiles */
     unsigned int *cnt_out;        /* survivors per output tile */
-    unsigned int *ticket;         /* zero before the launch */
+    unsigned int *ticket;         /* tile tickets (several populations); zero before the launch */
     int *error;                   /* set if the host's tile bound was too small */
     int ntiles;                   /* tiles launched (upper bound of the needed ones) */
     int new_in_block;
