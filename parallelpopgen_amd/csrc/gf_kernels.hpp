@@ -62,17 +62,26 @@ static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *_
         }
         __syncthreads();
         int run = carry + (w > 0 ? sh_w[w - 1] : 0) + x - sum;
+        int pv[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const int idx = b + i;
+            pv[i] = run;
             if (idx < T) {
                 const int c = (int)v[i];
-                pre[idx] = run;
                 if (c > 0) {
                     for (int j = (run + WG - 1) / WG; j * WG <= run + c - 1; j++) src0[j] = idx;
                 }
                 run += c;
             }
+        }
+        if (b + 16 <= T) {   /* four 16-byte stores (b is a multiple of 16) */
+            int4 *q = reinterpret_cast<int4 *>(pre + b);
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = make_int4(pv[4 * k], pv[4 * k + 1], pv[4 * k + 2], pv[4 * k + 3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) if (b + i < T) pre[b + i] = pv[i];
         }
         carry += sh_w[15];
         __syncthreads();

@@ -262,7 +262,7 @@ struct PlainRun {
         const int64_t nt = ncap / WG;
         if (nt > tiles_cap) {
             GF_TRY(d_cnt.alloc((size_t)nt));
-            GF_TRY(d_pre.alloc(2 * (size_t)(nt + 1)));
+            GF_TRY(d_pre.alloc(2 * (size_t)pre_stride(nt)));
             GF_TRY(d_src0.alloc(2 * (size_t)nt));
             tiles_cap = nt;
         }
@@ -275,7 +275,7 @@ struct PlainRun {
         if (!frag) return GF_OK;
         const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((M_bound + WG - 1) / WG, (int64_t)num_cus() * 8));
         hipLaunchKernelGGL((k_densify<P>), dim3((unsigned)nb), dim3(WG), 0, st, freqA.p, idsA.p, freqB.p, idsB.p, cap,
-                           d_pre.p + (size_t)par * (tiles_cap + 1), d_src0.p + (size_t)par * tiles_cap, d_T.p + par,
+                           d_pre.p + (size_t)par * pre_stride(tiles_cap), d_src0.p + (size_t)par * tiles_cap, d_T.p + par,
                            d_M.p + mcur);
         GF_HIP(hipGetLastError());
         freqA.swap(freqB);
@@ -478,6 +478,10 @@ struct PlainRun {
         return GF_OK;
     }
 
+    /* entries of each half of d_pre: T + 1 prefix values, rounded up to a
+     * multiple of 4 so that both halves take 16-byte stores */
+    static size_t pre_stride(int64_t tiles) { return ((size_t)tiles + 1 + 3) & ~(size_t)3; }
+
     /* grow the buffers to hold `need` slots; the current contents (exact
      * count read from the device) move along */
     int grow(int64_t need) {
@@ -502,7 +506,7 @@ struct PlainRun {
             bound = M_bound + new_in_block;
         }
         const int ntiles = (int)((bound + WG - 1) / WG);
-        const size_t pre_half = (size_t)tiles_cap + 1, src_half = (size_t)tiles_cap;
+        const size_t pre_half = pre_stride(tiles_cap), src_half = (size_t)tiles_cap;
         FusedLaunch F{};
         F.freq_in = freqA.p; F.ids_in = idsA.p; F.freq_out = freqB.p; F.ids_out = idsB.p;
         F.pitch = cap;
