@@ -61,6 +61,15 @@ static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *_
             if (lane < 16) sh_w[lane] = t;
         }
         __syncthreads();
+        if (threadIdx.x == 0 && base + 16 * 1024 >= T) {
+            /* the last round: the totals are known, and their stores (one
+             * of them to host memory) overlap the prefix stores below */
+            const int total = carry + sh_w[15];
+            pre[T] = total;
+            *M_out = total;
+            *T_out = T;
+            if (M_host) *M_host = total;   /* pinned host word: the host reads it after the launch's event */
+        }
         int run = carry + (w > 0 ? sh_w[w - 1] : 0) + x - sum;
         int pv[16];
 #pragma unroll
@@ -86,11 +95,11 @@ static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *_
         carry += sh_w[15];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        pre[T] = carry;
-        *M_out = carry;
-        *T_out = T;
-        if (M_host) *M_host = carry;   /* pinned host word: the host reads it after the launch's event */
+    if (T == 0 && threadIdx.x == 0) {
+        pre[0] = 0;
+        *M_out = 0;
+        *T_out = 0;
+        if (M_host) *M_host = 0;
     }
 }
 
