@@ -9,6 +9,7 @@
 #include <stdint.h>
 #include <mutex>
 #include <thread>
+#include <condition_variable>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -95,38 +96,98 @@ struct PinBuf {
     }
 };
 
-/* host memcpy split over a few threads (one core copies ~10 GB/s) */
-inline void parallel_copy(char *dst, const char *src, size_t bytes) {
-    const size_t min_part = (size_t)2 << 20;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t parts = std::min<size_t>(std::min<size_t>(8, hw), std::max<size_t>(1, bytes / min_part));
-    if (parts <= 1) { std::memcpy(dst, src, bytes); return; }
-    const size_t step = (bytes + parts - 1) / parts;
-    std::vector<std::thread> th;
-    for (size_t k = 1; k < parts; k++) {
-        const size_t off = k * step;
-        if (off >= bytes) break;
-        th.emplace_back([=] { std::memcpy(dst + off, src + off, std::min(step, bytes - off)); });
+/* host memcpy split over a few threads (one core copies ~10 GB/s): the
+ * calling thread and up to 7 pool workers, started once and kept until the
+ * process ends */
+class CopyPool {
+public:
+    static CopyPool &get() {
+        static CopyPool *pool = new CopyPool();   /* never destroyed: workers outlive static teardown */
+        return *pool;
     }
-    std::memcpy(dst, src, std::min(step, bytes));
-    for (auto &t : th) t.join();
-}
+    void copy(char *dst, const char *src, size_t bytes) {
+        const size_t min_part = (size_t)1 << 20;
+        const size_t parts = std::min<size_t>(workers_.size() + 1, std::max<size_t>(1, bytes / min_part));
+        if (parts <= 1) { std::memcpy(dst, src, bytes); return; }
+        const size_t step = (bytes + parts - 1) / parts;
+        std::lock_guard<std::mutex> one_job(job_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = dst; src_ = src; bytes_ = bytes; step_ = step;
+            next_ = 1; parts_ = parts; pending_ = parts - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        std::memcpy(dst, src, std::min(step, bytes));
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    CopyPool() {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned n = std::min(7u, hw > 1 ? hw - 1 : 0u);
+        for (unsigned k = 0; k < n; k++) workers_.emplace_back([this] { run(); });
+        for (auto &t : workers_) t.detach();
+    }
+    void run() {
+        uint64_t seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return gen_ != seen && next_ < parts_; });
+            const size_t k = next_++;
+            if (next_ >= parts_) seen = gen_;
+            char *d = dst_;
+            const char *s = src_;
+            const size_t off = k * step_, b = off < bytes_ ? std::min(step_, bytes_ - off) : 0;
+            lk.unlock();
+            if (b) std::memcpy(d + off, s + off, b);
+            lk.lock();
+            if (--pending_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex job_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0, step_ = 0, next_ = 0, parts_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+};
+
+inline void parallel_copy(char *dst, const char *src, size_t bytes) { CopyPool::get().copy(dst, src, bytes); }
 
 /* Device -> host copy through a process-wide pinned staging buffer (kept
  * between calls, grown on demand): a copy into pageable memory runs at a
- * fraction of the link's bandwidth.  Serialised by a mutex. */
+ * fraction of the link's bandwidth.  Two halves: the link fills one while
+ * the host copies the other out.  Serialised by a mutex. */
 inline int readback(void *dst, const void *src, size_t bytes, hipStream_t st) {
     static std::mutex mu;
     static PinBuf<char> stage;
     if (bytes == 0) return GF_OK;
     std::lock_guard<std::mutex> lock(mu);
-    const size_t chunk = (size_t)64 << 20;
-    if (stage.n < std::min(bytes, chunk)) GF_TRY(stage.alloc(std::min(bytes, chunk)));
-    for (size_t off = 0; off < bytes; off += stage.n) {
-        const size_t b = std::min(stage.n, bytes - off);
-        GF_HIP(hipMemcpyAsync(stage.p, (const char *)src + off, b, hipMemcpyDeviceToHost, st));
-        GF_HIP(hipStreamSynchronize(st));
-        parallel_copy((char *)dst + off, stage.p, b);
+    const size_t half = std::min(bytes, (size_t)16 << 20);
+    if (stage.n < 2 * half) GF_TRY(stage.alloc(2 * half));
+    const size_t h = stage.n / 2;
+    struct Events {   /* created on the stream's device, destroyed on return */
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+    } ev;
+    hipEvent_t *filled = ev.e;
+    for (int k = 0; k < 2; k++) GF_HIP(hipEventCreateWithFlags(&filled[k], hipEventDisableTiming));
+    const size_t nchunks = (bytes + h - 1) / h;
+    auto issue = [&](size_t c) -> int {
+        const size_t off = c * h, b = std::min(h, bytes - off);
+        GF_HIP(hipMemcpyAsync(stage.p + (c & 1) * h, (const char *)src + off, b, hipMemcpyDeviceToHost, st));
+        GF_HIP(hipEventRecord(filled[c & 1], st));
+        return GF_OK;
+    };
+    GF_TRY(issue(0));
+    for (size_t c = 0; c < nchunks; c++) {
+        if (c + 1 < nchunks) GF_TRY(issue(c + 1));   /* the other half was copied out in the previous round */
+        GF_HIP(hipEventSynchronize(filled[c & 1]));
+        const size_t off = c * h;
+        parallel_copy((char *)dst + off, stage.p + (c & 1) * h, std::min(h, bytes - off));
     }
     return GF_OK;
 }
