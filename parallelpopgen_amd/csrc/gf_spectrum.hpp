@@ -114,7 +114,16 @@ __global__ __launch_bounds__(WG) void k_sfs(const float *__restrict__ freq, cons
 void log_binom_table(int n, std::vector<float> &lc, int &half_n);
 void finish_sfs(const unsigned long long *acc, unsigned long long nonseg, int n, int M, float num_sites, float *out);
 /* sample SFS of one population row already in device memory (M floats) */
-int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st);
+/* device buffers of the sample SFS of one row, kept between calls by their
+ * owner (grow-only) */
+struct SfsScratch {
+    DevBuf<float> lc;
+    DevBuf<unsigned long long> acc;
+    DevBuf<int> meta;
+    DevBuf<int64_t> base;
+};
+int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st,
+                      SfsScratch *scratch = nullptr);
 int sfs_device(const float *d_freq, const int64_t *d_seg_base, const int *d_seg_M, const int *d_tile_seg,
                const int *d_tile_first, int num_tiles, int nseg, int n, unsigned long long *d_acc,
                hipStream_t st, DevBuf<float> &d_lc);

@@ -43,6 +43,12 @@ struct gf_result {
     int dev_device = 0;
     std::vector<int64_t> live;
     gf_timing timing{};
+    /* stream and scratch of the device-side SFS calls on this result */
+    hipStream_t sfs_stream = nullptr;
+    gf::SfsScratch sfs_scratch;
+    ~gf_result() {
+        if (sfs_stream) (void)hipStreamDestroy(sfs_stream);
+    }
 };
 
 namespace gf {
@@ -834,10 +840,10 @@ int32_t gf_result_site_frequency_spectrum(const gf_result *r, int32_t s, int32_t
     int cur = 0;
     GF_HIP(hipGetDevice(&cur));
     if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    const int rc = gf::sfs_of_device_row(r->dev_freq.p + (size_t)pop * r->dev_pitch, x.num_mutations, n, num_sites, out, st);
-    (void)hipStreamDestroy(st);
+    gf_result *rw = const_cast<gf_result *>(r);   /* the result's own SFS scratch, not its data */
+    if (!rw->sfs_stream) GF_HIP(hipStreamCreateWithFlags(&rw->sfs_stream, hipStreamNonBlocking));
+    const int rc = gf::sfs_of_device_row(r->dev_freq.p + (size_t)pop * r->dev_pitch, x.num_mutations, n, num_sites, out,
+                                         rw->sfs_stream, &rw->sfs_scratch);
     if (cur != r->dev_device) (void)hipSetDevice(cur);
     return rc;
 }

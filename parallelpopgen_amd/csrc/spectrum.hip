@@ -89,11 +89,14 @@ int joint_sfs_of_device_rows(const float *d_fa, const float *d_fb, int M, int na
 }
 
 /* sample SFS of one population row already in device memory (M floats) */
-int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st) {
-    DevBuf<float> d_lc;
-    DevBuf<unsigned long long> d_acc;
-    DevBuf<int> d_meta;
-    DevBuf<int64_t> d_base;
+int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st,
+                      SfsScratch *scratch) {
+    SfsScratch local;
+    SfsScratch &sc = scratch ? *scratch : local;
+    DevBuf<float> &d_lc = sc.lc;
+    DevBuf<unsigned long long> &d_acc = sc.acc;
+    DevBuf<int> &d_meta = sc.meta;
+    DevBuf<int64_t> &d_base = sc.base;
     const int tiles = (M + WG - 1) / WG;
     GF_TRY(d_acc.alloc((size_t)n + 2));
     GF_TRY(d_meta.alloc((size_t)tiles + 2));
