@@ -537,7 +537,7 @@ int32_t gf_sg_train(gf_sg_model *S, const float *x, const float *y, const float 
  * mean log-total and the ensemble disagreement per input */
 int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, float *tot, float *dis) {
     if (!S || !x || n <= 0 || !sfs) { set_error("gf_sg_predict: invalid arguments"); return GF_ERR_ARG; }
-    const int B = std::min(8192, ((n + TILE - 1) / TILE) * TILE);
+    const int B = std::min(65536, ((n + TILE - 1) / TILE) * TILE);
     GF_TRY(ensure_batch(*S, B));
     std::vector<float> xb((size_t)B * 6, 0.0f);
     for (int off = 0; off < n; off += B) {
@@ -549,7 +549,8 @@ int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, flo
         hipLaunchKernelGGL(k_predict_stats, dim3((B + 63) / 64), dim3(64), 0, S->st, S->logits.p, (int64_t)B * OUT_PAD,
                            S->E, B, S->psfs.p, S->ptot.p, S->pdis.p);
         GF_HIP(hipGetLastError());
-        GF_HIP(hipMemcpyAsync(sfs + (size_t)off * NBINS, S->psfs.p, (size_t)cnt * NBINS * sizeof(float), hipMemcpyDeviceToHost, S->st));
+        /* the spectra (99 floats per input) through the pinned staging path */
+        GF_TRY(readback(sfs + (size_t)off * NBINS, S->psfs.p, (size_t)cnt * NBINS * sizeof(float), S->st));
         if (tot) GF_HIP(hipMemcpyAsync(tot + off, S->ptot.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, S->st));
         if (dis) GF_HIP(hipMemcpyAsync(dis + off, S->pdis.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, S->st));
         GF_HIP(hipStreamSynchronize(S->st));
