@@ -43,9 +43,11 @@ struct gf_result {
     int dev_device = 0;
     std::vector<int64_t> live;
     gf_timing timing{};
-    /* stream and scratch of the device-side SFS calls on this result */
-    hipStream_t sfs_stream = nullptr;
-    gf::SfsScratch sfs_scratch;
+    /* stream and scratch of the device-side SFS calls on this result (not
+     * its data: mutable through the const accessors; like the reference's
+     * API, one result is used by one host thread at a time) */
+    mutable hipStream_t sfs_stream = nullptr;
+    mutable gf::SfsScratch sfs_scratch;
     ~gf_result() {
         if (sfs_stream) (void)hipStreamDestroy(sfs_stream);
     }
@@ -840,10 +842,9 @@ int32_t gf_result_site_frequency_spectrum(const gf_result *r, int32_t s, int32_t
     int cur = 0;
     GF_HIP(hipGetDevice(&cur));
     if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
-    gf_result *rw = const_cast<gf_result *>(r);   /* the result's own SFS scratch, not its data */
-    if (!rw->sfs_stream) GF_HIP(hipStreamCreateWithFlags(&rw->sfs_stream, hipStreamNonBlocking));
+    if (!r->sfs_stream) GF_HIP(hipStreamCreateWithFlags(&r->sfs_stream, hipStreamNonBlocking));
     const int rc = gf::sfs_of_device_row(r->dev_freq.p + (size_t)pop * r->dev_pitch, x.num_mutations, n, num_sites, out,
-                                         rw->sfs_stream, &rw->sfs_scratch);
+                                         r->sfs_stream, &r->sfs_scratch);
     if (cur != r->dev_device) (void)hipSetDevice(cur);
     return rc;
 }
