@@ -400,7 +400,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     __shared__ unsigned int sh_wcount[WG / 64];
     __shared__ int sh_pre[WG + 1];
     __shared__ unsigned long long sh_live[64];
+    __shared__ int sh_diff[KMAX + 1];
     if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
+    if (threadIdx.x <= KMAX) sh_diff[threadIdx.x] = 0;
     const int M0 = *F.M_in;
     const int Mend = M0 + F.new_in_block;
     const int need_tiles = (Mend + WG - 1) / WG;
@@ -464,6 +466,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
         }
         u32x4 rc[P];
         bool keep = false;
+        int kfirst = KMAX, klast = -1;
         const auto *crows = as_const(rows);
         for (int t = 0; t < S.K; t++) {
             const auto &R = crows[t];
@@ -473,8 +476,22 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
                 generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel);
             }
             keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
-            const uint64_t b = __ballot(keep);
-            if (lane == t) cnt += (uint64_t)__popcll(b);
+            if constexpr (P == 1) {
+                /* one population: a slot is kept over one run of generations
+                 * (lost and fixed are absorbing), recorded by its ends */
+                if (keep) { kfirst = t < kfirst ? t : kfirst; klast = t; }
+            } else {
+                const uint64_t b = __ballot(keep);
+                if (lane == t) cnt += (uint64_t)__popcll(b);
+            }
+        }
+        if constexpr (P == 1) {
+            /* per-generation live counts as a difference array: +1 at the
+             * run's first generation, -1 after its last */
+            if (klast >= 0) {
+                atomicAdd(&sh_diff[kfirst], 1);
+                if (klast + 1 < S.K) atomicAdd(&sh_diff[klast + 1], -1);
+            }
         }
         /* without compaction every slot of the block stays */
         if (!F.compact) keep = valid;
@@ -506,7 +523,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
         }
     }
     /* per-generation live counts: lane t of every wave holds generation t */
-    if (cnt) atomicAdd(&sh_live[lane], (unsigned long long)cnt);
+    if constexpr (P == 1) {
+        __syncthreads();
+        if (threadIdx.x == 0) {   /* prefix of the difference array */
+            long long run = 0;
+            for (int t = 0; t < S.K; t++) {
+                run += sh_diff[t];
+                sh_live[t] = (unsigned long long)run;
+            }
+        }
+    } else {
+        if (cnt) atomicAdd(&sh_live[lane], (unsigned long long)cnt);
+    }
     __syncthreads();
     if ((int)threadIdx.x < S.K && sh_live[threadIdx.x])
         atomicAdd(&S.live[(size_t)(S.t0 + threadIdx.x) * 64 + (blockIdx.x & 63)], sh_live[threadIdx.x]);
