@@ -173,14 +173,37 @@ __global__ void k_loss(const float *__restrict__ logits, int64_t sL, const float
     atomicAdd(&loss[z], (ce + w_tot * e * e) * invB);
 }
 
-/* column sums of a bf16 [B][N] matrix -> fp32 [N] (bias gradients) */
-__global__ void k_colsum(const bf16 *__restrict__ dz, int64_t sD, int B, int N, float *__restrict__ out, int64_t sO) {
+/* column sums of a bf16 [B][N] matrix -> fp32 [N] (bias gradients), N a
+ * multiple of 64.  A workgroup sums 64 columns: 32 row lanes, each of 32
+ * threads reading two adjacent columns (one 4-byte load) of every 32nd
+ * row, then the 32 partial sums of a column are added in a fixed order, so
+ * the result does not depend on timing. */
+constexpr int COLSUM_LANES = 32;
+static_assert(HID % 64 == 0 && OUT_PAD % 64 == 0, "k_colsum sums 64 columns per workgroup");
+__global__ __launch_bounds__(1024) void k_colsum(const bf16 *__restrict__ dz, int64_t sD, int B, int N,
+                                                 float *__restrict__ out, int64_t sO) {
+    __shared__ float part[COLSUM_LANES][64 + 1];
     const int z = blockIdx.y;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.f;
-    for (int b = 0; b < B; b++) s += (float)dz[z * sD + (int64_t)b * N + n];
-    out[z * sO + n] = s;
+    const int cp = (int)threadIdx.x & 31;
+    const int rl = (int)threadIdx.x >> 5;
+    const int n0 = (int)blockIdx.x * 64;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(dz + z * sD + n0) + cp;
+    const int64_t row = N / 2;   /* in 4-byte words */
+    float s0 = 0.f, s1 = 0.f;
+#pragma unroll 4
+    for (int b = rl; b < B; b += COLSUM_LANES) {
+        const uint32_t v = src[(int64_t)b * row];
+        s0 += __builtin_bit_cast(float, v << 16);            /* column n0 + 2 cp */
+        s1 += __builtin_bit_cast(float, v & 0xffff0000u);    /* column n0 + 2 cp + 1 */
+    }
+    part[rl][2 * cp] = s0;
+    part[rl][2 * cp + 1] = s1;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float s = 0.f;
+        for (int k = 0; k < COLSUM_LANES; k++) s += part[k][threadIdx.x];
+        out[z * sO + n0 + (int)threadIdx.x] = s;
+    }
 }
 
 /* Adam on fp32 master weights; refreshes the bf16 copy */
@@ -362,7 +385,8 @@ static int train_step(gf_sg_model &S, int B, float lr, float w_tot, bool update)
         a.M = O; a.N = I; a.K = B;
         a.epi = EPI_DW; a.Cf = S.g.p + L.W[l]; a.ldc = I; a.sCf = sW;
         GF_TRY(gemm(S, a));
-        hipLaunchKernelGGL(k_colsum, dim3((O + 127) / 128, E), dim3(128), 0, S.st, dcur, sD, B, O, S.g.p + L.b[l], sW);
+        hipLaunchKernelGGL(k_colsum, dim3(O / 64, E), dim3(32 * COLSUM_LANES), 0, S.st, dcur, sD, B, O,
+                           S.g.p + L.b[l], sW);
         GF_HIP(hipGetLastError());
         if (l == 0) break;
         /* dZ_(l-1) = (dY W_l) * gelu'(Z_(l-1)) : M = B, N = I, K = O */
