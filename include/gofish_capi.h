@@ -297,7 +297,7 @@ const char *gf_last_error(void);
 int32_t gf_set_kernel_timing(int32_t enable);
 /* Number of generations the drift kernel advances per launch at most
  * (temporal blocking depth); 1 gives one generation per launch, 0 restores
- * the default (10 for one population, 16 otherwise and for batches). */
+ * the default (10 for one population, 20 for several, 16 for batches). */
 int32_t gf_set_generations_per_launch(int32_t k);
 int32_t gf_device_count(int32_t *out);
 

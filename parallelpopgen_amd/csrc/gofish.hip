@@ -634,8 +634,10 @@ struct PlainRun {
         int t = 0;   /* generations done */
         /* one population compacts at every launch: shorter launches carry
          * fewer lost slots (10 measured best for N = 1e5, theta = 2e5);
-         * several populations end launches at the compaction interval */
-        const int kpl = std::max(1, std::min(KMAX, generations_per_launch(P == 1 ? 10 : 16)));
+         * several populations end launches at the compaction interval, 20
+         * generations otherwise (three populations of 2e4-2e5 over 2,000
+         * generations: 16 per launch 158 ms, 20 146 ms, 24 146 ms) */
+        const int kpl = std::max(1, std::min(KMAX, generations_per_launch(P == 1 ? 10 : 20)));
         /* one population: a lost or fixed mutation never changes again, so
          * compacting every launch leaves every reported sample unchanged */
         const bool compact_every_launch = (P == 1 && interval > 0);
