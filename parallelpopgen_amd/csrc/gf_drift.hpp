@@ -221,7 +221,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int w = threadIdx.x >> 6;
     __shared__ int sh_tile;
     __shared__ unsigned int sh_wc[WG / 64];
-    uint64_t cnt = 0;
+    uint64_t cnt = 0;        /* one population: lane t counts generation t */
+    uint64_t wave_cnt = 0;   /* batches: the wave's live mutation-generations */
     /* tiles are handed out dynamically (an atomic ticket): their cost varies
      * with the simulation they belong to */
     for (;;) {
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             const uint64_t b = __ballot(keep);
             if (BATCH) {
-                if (lane == 0) cnt += (uint64_t)__popcll(b);
+                wave_cnt += (uint64_t)__popcll(b);   /* wave-uniform: scalar registers */
             } else {
                 if (lane == t) cnt += (uint64_t)__popcll(b);
             }
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
         /* every segment counts into the same 64 counters (seg0.live): one
          * atomic per wave at the end, spread over 64 addresses (a per-tile
          * atomic on one address held every wave at the next barrier) */
-        if (lane == 0 && cnt) atomicAdd(&seg0.live[blockIdx.x & 63], (unsigned long long)cnt);
+        if (lane == 0 && wave_cnt) atomicAdd(&seg0.live[blockIdx.x & 63], (unsigned long long)wave_cnt);
     } else {
         __shared__ unsigned long long sh[64];
         if (threadIdx.x < 64) sh[threadIdx.x] = 0;
