@@ -45,34 +45,149 @@ struct Status {
         if (rc_ != GF_OK) return rc_;    \
     } while (0)
 
-/* owning device buffer */
+/* Per-device cache of released device blocks.  hipFree waits for the whole
+ * device and took up to 80 ms for large blocks (a batched run releases
+ * about 60 buffers); a released block is kept instead, after the same
+ * device-wide wait, so no block is handed out while earlier work may still
+ * use it.  An allocation takes the smallest cached block of its size class
+ * up to twice its size.  At most kKeep bytes are kept per device (the
+ * oldest blocks are freed first); blocks above kMaxBlock are freed at
+ * once; a failed hipMalloc empties the device's cache and tries again.
+ * The cache lives until the process ends. */
+class DevCache {
+public:
+    static constexpr size_t kKeep = (size_t)16 << 30;
+    static constexpr size_t kMaxBlock = (size_t)4 << 30;
+
+    static DevCache &get() {
+        static DevCache *c = new DevCache();   /* never destroyed: no teardown-order hazard */
+        return *c;
+    }
+
+    /* size class: powers of two from 256 B below 1 MiB, 2 MiB steps above */
+    static size_t size_class(size_t bytes) {
+        if (bytes >= ((size_t)1 << 20)) return (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+
+    hipError_t alloc(void **p, size_t bytes, size_t &block, int &dev) {
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        block = size_class(bytes);
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            Dev &D = dev_(dev);
+            size_t best = D.blocks.size();
+            for (size_t i = 0; i < D.blocks.size(); i++) {
+                const size_t b = D.blocks[i].bytes;
+                if (b >= block && b <= 2 * block && (best == D.blocks.size() || b < D.blocks[best].bytes)) best = i;
+            }
+            if (best < D.blocks.size()) {
+                *p = D.blocks[best].p;
+                block = D.blocks[best].bytes;
+                D.kept -= block;
+                D.blocks.erase(D.blocks.begin() + (ptrdiff_t)best);
+                return hipSuccess;
+            }
+        }
+        e = hipMalloc(p, block);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            flush(dev);
+            e = hipMalloc(p, block);
+        }
+        return e;
+    }
+
+    void release(void *p, size_t block, int dev) {
+        int cur = dev;
+        (void)hipGetDevice(&cur);
+        if (cur != dev) (void)hipSetDevice(dev);
+        (void)hipDeviceSynchronize();
+        if (block > kMaxBlock) {
+            (void)hipFree(p);
+        } else {
+            std::vector<void *> drop;
+            {
+                std::lock_guard<std::mutex> lock(mu_);
+                Dev &D = dev_(dev);
+                D.blocks.push_back({p, block});
+                D.kept += block;
+                while (D.kept > kKeep && !D.blocks.empty()) {
+                    drop.push_back(D.blocks.front().p);
+                    D.kept -= D.blocks.front().bytes;
+                    D.blocks.erase(D.blocks.begin());
+                }
+            }
+            for (void *q : drop) (void)hipFree(q);
+        }
+        if (cur != dev) (void)hipSetDevice(cur);
+    }
+
+private:
+    struct Block { void *p; size_t bytes; };
+    struct Dev { std::vector<Block> blocks; size_t kept = 0; };
+    std::mutex mu_;
+    std::vector<Dev> devs_;
+
+    Dev &dev_(int d) {
+        if ((size_t)d >= devs_.size()) devs_.resize((size_t)d + 1);
+        return devs_[(size_t)d];
+    }
+
+    /* frees every cached block of device `dev` (the current device) */
+    void flush(int dev) {
+        std::vector<Block> drop;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            Dev &D = dev_(dev);
+            drop.swap(D.blocks);
+            D.kept = 0;
+        }
+        (void)hipDeviceSynchronize();
+        for (const Block &b : drop) (void)hipFree(b.p);
+    }
+};
+
+/* owning device buffer (n elements; the block under it may be larger) */
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    size_t block = 0;
+    int dev = 0;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) DevCache::get().release(p, block, dev);
         p = nullptr;
         n = 0;
+        block = 0;
     }
     int alloc(size_t count) {
         if (count <= n && p) return GF_OK;
         release();
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        hipError_t e = DevCache::get().alloc((void **)&p, count * sizeof(T), block, dev);
         if (e != hipSuccess) {
             p = nullptr;
+            block = 0;
             set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
             return GF_ERR_ALLOC;
         }
         n = count;
         return GF_OK;
     }
-    void swap(DevBuf &o) { std::swap(p, o.p); std::swap(n, o.n); }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(block, o.block);
+        std::swap(dev, o.dev);
+    }
 };
 
 /* pinned host buffer */
