@@ -281,20 +281,16 @@ GF_HD float exp_f_mid(float x) {
     return p * pow2i((int)n);
 }
 
-/* one term of the Poisson CDF recurrence; true if the inversion stops at it */
-GF_HD bool pois_term(float &emu, float &cdf, float mean, float rj, float r, int j, int jstop) {
-    GF_STRICT_FP
-    emu = emu * (mean * rj);
-    cdf = cdf + emu;
-    return cdf >= r || j == jstop;
-}
-
 /* Poisson CDF inversion truncated as in the reference (shared.cuh:168-249):
  * the first j with CDF(j) >= r, or the early-exit index of `mean` if that
  * comes first, or 70.  Callers pass 0 <= mean <= 6 (the Poisson tails of
  * the approximate binomial), where exp_f_mid(-mean) == exp_f(-mean).  The
- * loop stays rolled (lanes leave it as they finish, with no nested branch
- * per term) and takes four reciprocals per trip in one scalar load. */
+ * loop stays rolled (lanes leave it as they finish) and takes four
+ * reciprocals per trip in one scalar load; a trip computes its four terms
+ * straight through and then picks the first that stops, so a wave changes
+ * its lane mask once per trip instead of once per term (the terms past the
+ * stopping one are computed and discarded; the recurrence and the
+ * comparisons are those of one term at a time). */
 GF_HD int poiscdfinv(float r, float mean) {
     GF_STRICT_FP
     float emu = exp_f_mid(-1.0f * mean);
@@ -304,11 +300,19 @@ GF_HD int poiscdfinv(float r, float mean) {
 #pragma nounroll
     for (int j = 1;; j += 4) {
         const float c0 = kRecip[j - 1], c1 = kRecip[j], c2 = kRecip[j + 1], c3 = kRecip[j + 2];
-        if (pois_term(emu, cdf, mean, c0, r, j, jstop)) return j;
-        if (j == 69) return 70;
-        if (pois_term(emu, cdf, mean, c1, r, j + 1, jstop)) return j + 1;
-        if (pois_term(emu, cdf, mean, c2, r, j + 2, jstop)) return j + 2;
-        if (pois_term(emu, cdf, mean, c3, r, j + 3, jstop)) return j + 3;
+        const float e0 = emu * (mean * c0), d0 = cdf + e0;
+        const float e1 = e0 * (mean * c1), d1 = d0 + e1;
+        const float e2 = e1 * (mean * c2), d2 = d1 + e2;
+        const float e3 = e2 * (mean * c3), d3 = d2 + e3;
+        const int rel = jstop - j;   /* the early exit, relative to this trip */
+        int k = (d3 >= r || rel == 3) ? 3 : 4;
+        k = (d2 >= r || rel == 2) ? 2 : k;
+        k = (d1 >= r || rel == 1) ? 1 : k;
+        if (j == 69) k = (d0 >= r || rel == 0) ? 0 : 1;   /* term 69 is the last: 70 after it */
+        else k = (d0 >= r || rel == 0) ? 0 : k;
+        if (k < 4) return j + k;
+        emu = e3;
+        cdf = d3;
     }
 }
 
