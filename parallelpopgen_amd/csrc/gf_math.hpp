@@ -140,6 +140,25 @@ GF_HD float pow_f(float a, float b) {
     return exp_f(b * log_f(a));
 }
 
+/* Correctly rounded square root of a positive normal float x >= 2^-96 (the
+ * variance of the normal approximation, at least 36/N there, and w >= 8 in
+ * the inverse normal's tail).  On the
+ * device: the hardware estimate (within one ulp) moved by one ulp where the
+ * fused residual x - y*y says so, which is the compiler's own correctly
+ * rounded expansion without its scaling of tiny inputs and its zero,
+ * infinity and NaN cases; the host's sqrtf is correctly rounded. */
+GF_HD float sqrt_pos(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y = __builtin_amdgcn_sqrtf(x);
+    const float ydn = bits2f(f2bits(y) - 1u), yup = bits2f(f2bits(y) + 1u);
+    float r = __builtin_fmaf(-ydn, y, x) <= 0.0f ? ydn : y;
+    r = __builtin_fmaf(-yup, y, x) > 0.0f ? yup : r;
+    return r;
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
+
 /* Inverse standard normal CDF for p in [2^-33, 1] (the range of u01; p is
  * clamped to <= 1 - 2^-24 so the result stays finite).  Method: the change
  * of variable of Giles ("Approximating the erfinv function", GPU Computing
@@ -183,7 +202,7 @@ GF_HD float normcdfinv_f(float p) {
     const float w = -(float)e - f * horner_fma(kNcdfLog2, f);   /* -log2(v) >= 0 */
     float r;
     if (w < 8.0f) r = horner_fma(kNcdfCentral, w - 4.0f);
-    else r = horner_fma(kNcdfTail, __builtin_sqrtf(w) - 0x1.0cff86p+2f);
+    else r = horner_fma(kNcdfTail, sqrt_pos(w) - 0x1.0cff86p+2f);
     return q * r;
 }
 
@@ -296,7 +315,12 @@ GF_HD int poiscdfinv(float r, float mean) {
     float emu = exp_f_mid(-1.0f * mean);
     float cdf = emu;
     if (cdf >= r) return 0;
-    const int jstop = exit_index(mean);
+    /* exit_index for mean <= 14 without its branches (callers pass <= 6) */
+    const float cm = __builtin_ceilf(mean);
+    const int km = cm < 1.0f ? 1 : (int)cm;
+    const int over = km > 4 ? km - 4 : 0;
+    int jstop = 8 + 3 * km - over;   /* 8 + 3 km up to 4, 12 + 2 km beyond */
+    if (!(mean <= 14.0f)) jstop = exit_index(mean);
 #pragma nounroll
     for (int j = 1;; j += 4) {
         const float c0 = kRecip[j - 1], c1 = kRecip[j], c2 = kRecip[j + 1], c3 = kRecip[j + 2];
@@ -333,24 +357,6 @@ static __host__ __device__ __attribute__((noinline)) int binomcdfinv(float r, fl
         if (em >= 0 && mean <= (float)em) return j;
     }
     return 70;
-}
-
-/* Correctly rounded square root of a positive normal float x >= 2^-96 (the
- * variance of the normal approximation, at least 36/N there).  On the
- * device: the hardware estimate (within one ulp) moved by one ulp where the
- * fused residual x - y*y says so, which is the compiler's own correctly
- * rounded expansion without its scaling of tiny inputs and its zero,
- * infinity and NaN cases; the host's sqrtf is correctly rounded. */
-GF_HD float sqrt_pos(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const float y = __builtin_amdgcn_sqrtf(x);
-    const float ydn = bits2f(f2bits(y) - 1u), yup = bits2f(f2bits(y) + 1u);
-    float r = __builtin_fmaf(-ydn, y, x) <= 0.0f ? ydn : y;
-    r = __builtin_fmaf(-yup, y, x) > 0.0f ? yup : r;
-    return r;
-#else
-    return __builtin_sqrtf(x);
-#endif
 }
 
 /* approximate binomial draw (reference shared.cu:10-18): the two Poisson
