@@ -1,7 +1,8 @@
 """Edge cases of the GPU run_sim path against the oracle: a run in which no
 mutation ever arises, the maximum of 8 populations, a time sample in every
-generation, and the reference's parameter checks (check_sim_parameters,
-go_fish_impl.cuh:431-452) rejecting a run before anything is simulated."""
+generation, the reference's parameter checks (check_sim_parameters,
+go_fish_impl.cuh:431-452) rejecting a run before anything is simulated, and
+runs on device memory that earlier calls released."""
 import numpy as np
 import pytest
 
@@ -71,3 +72,30 @@ def test_migration_must_sum_to_one(gpu):
     with pytest.raises(GoFishError) as e:
         gpu.run(d)
     assert "migration rate does not sum to 1" in str(e.value)
+
+
+def test_reused_device_memory(gpu, oracle):
+    """device memory a call releases is kept by the library and handed to
+    later calls with whatever it held: a larger run with other values
+    between two identical runs must not change the second one"""
+    d = S.plain_small(G=120)
+    a = gpu.run(d)
+    big = S.constant(N=3000, L=2e8, G=60, P=3, m=1e-3, s=S.selection_from_gamma(4.0, 3000, 0.0), h=0.2,
+                     sample_every=20)
+    gpu.run(big)
+    b = gpu.run(d)
+    assert_same(b, a, "after a larger run")
+    assert_same(b, oracle.run(d), "oracle")
+
+
+def test_reused_device_memory_batched(gpu):
+    from test_gpu_batch import make_batch
+    sims = make_batch(12, np.random.default_rng(5))
+    a = gpu.batch_run(sims, compact_interval=16, sfs_n=30, keep_states=True)
+    gpu.batch_run(make_batch(40, np.random.default_rng(6), L_scale=4.0), compact_interval=8, sfs_n=50)
+    b = gpu.batch_run(sims, compact_interval=16, sfs_n=30, keep_states=True)
+    assert np.array_equal(a["sfs"].view(np.uint32), b["sfs"].view(np.uint32))
+    assert np.array_equal(a["num_mutations"], b["num_mutations"])
+    for s, ((fa, ia), (fb, ib)) in enumerate(zip(a["states"], b["states"])):
+        assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32)), s
+        assert np.array_equal(ia, ib), s

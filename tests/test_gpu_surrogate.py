@@ -50,7 +50,7 @@ def test_ensemble_members_differ_and_seed_reproduces():
     assert not np.array_equal(a[0], a[1]) and not np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("B", [64, 256])
+@pytest.mark.parametrize("B", [64, 192, 256])
 def test_gradients_match_oracle(B):
     net = SG.Network(ensemble=2, seed=12)
     x, y, t = inputs(B, 2)
