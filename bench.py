@@ -33,6 +33,14 @@ Other workloads (--workload), each printing one JSON line of its own:
   surrogate  trains the surrogate ensemble on batched simulations and
              reports its predictions/s, the batched simulator's SFS/s and
              the held-out deviance against two simulation replicates
+  speed      the reference's own speed benchmark (examples/example_speed/
+             run.cu:12-25: 2e5 haploid individuals, neutral, mu = 1e-9,
+             1,000 generations from equilibrium, L = xL * 2e7 sites) at
+             xL = 20 and 400, each with the compaction interval of the
+             reference's GTX 980 run; mean run_sim time over 10 runs after
+             10 warm-up runs, as the reference times it, against its
+             published times (tests/golden/speed_snps.json, from
+             documentation/780M_980_Intel_speed_results.xlsx)
 """
 import argparse
 import json
@@ -59,7 +67,7 @@ def parse():
     ap.add_argument("--generations", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-generations", type=int, default=0, help="generations of the CPU sample (0: auto)")
-    ap.add_argument("--workload", choices=["plain_large", "three_population", "batched", "surrogate"],
+    ap.add_argument("--workload", choices=["plain_large", "three_population", "batched", "surrogate", "speed"],
                     default="plain_large")
     ap.add_argument("--sims", type=int, default=4096, help="batched: simulations in the sweep")
     ap.add_argument("--plain-sample", type=int, default=32, help="batched: simulations timed back to back")
@@ -205,6 +213,52 @@ def bench_surrogate(a):
     }
 
 
+def bench_speed(a, capi, scenarios):
+    """The reference's speed benchmark at xL = 20 and 400 (value: xL = 400)."""
+    with open(os.path.join(HERE, "tests", "golden", "speed_snps.json")) as f:
+        pub = json.load(f)
+    runs = []
+    for xL in (20, 400):
+        r980 = [r for r in pub["980"] if abs(r["xL"] - xL) < 1e-9][0]
+        r780 = [r for r in pub["780M"] if abs(r["xL"] - xL) < 1e-9][0]
+        d = scenarios.constant(N=200000, F=1.0, h=0.0, s=0.0, mu=1e-9, L=xL * 2e7, G=1000,
+                               compact_interval=int(r980["compact_interval"]), seed1=0xbeeff00d, seed2=0xdecafbad)
+        for _ in range(10):
+            capi.run(d)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            r = capi.run(d)
+        ms = 1e3 * (time.perf_counter() - t0) / 10
+        mg = int(r["live"][1:].sum())
+        runs.append({"xL": xL, "compact_interval": int(r980["compact_interval"]), "run_sim_ms": ms,
+                     "snps": int(r["samples"][-1]["num_mutations"]), "mutation_generations": mg,
+                     "mutation_generations_per_s": mg / (ms * 1e-3),
+                     "gtx980_published_ms": r980["ms"], "gtx980_snps": r980["snps"],
+                     "gtx780m_published_ms": r780["ms"], "speedup_vs_gtx980": r980["ms"] / ms})
+    big = runs[-1]
+    return {
+        "metric": "run_sim ms (reference speed benchmark, xL=400)",
+        "value": big["run_sim_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": 10,
+        "warmup": 10,
+        "ms_per_step": big["run_sim_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": big["run_sim_ms"] / big["gtx980_published_ms"],
+        "dtype": "f32",
+        "data": "synthetic (simulator's own neutral-equilibrium start)",
+        "config": {
+            "workload": "speed: run_sim of 2e5 haploid individuals (F=1), neutral, mu=1e-9, 1000 generations "
+                        "from equilibrium, L=xL*2e7, compaction interval of the reference's GTX 980 run; "
+                        "mean of 10 runs after 10 warm-up runs",
+            "parallelism": "single GPU",
+        },
+        "runs": runs,
+    }
+
+
 def bench_three_population(a, capi, scenarios):
     """Three populations (exponential growth 2e4 -> 2e5, a bottleneck, F = 0.1,
     migration 1e-4, sine-wave selection, preserve every 100 generations),
@@ -268,6 +322,9 @@ def main():
         return
     if a.workload == "three_population":
         print(json.dumps(bench_three_population(a, capi, scenarios)))
+        return
+    if a.workload == "speed":
+        print(json.dumps(bench_speed(a, capi, scenarios)))
         return
     capi.set_kernel_timing(TIMING_STRIDE)
     for _ in range(a.warmup):
