@@ -12,21 +12,73 @@
 namespace gf {
 
 /* ---------------- prefix of the fused launch's tile counts ----------------
- * One workgroup.  T = tiles of the launch (from the device slot count
- * M_in + new); pre[s] = survivors of tiles before s, pre[T] = total =
- * *M_out; for every tile j of the next launch (256 j < total) src0[j] = the
- * tile holding compacted index 256 j. */
+ * T = tiles of the launch (from the device slot count M_in + new);
+ * pre[s] = survivors of tiles before s, pre[T] = total = *M_out; for every
+ * tile j of the next launch (256 j < total) src0[j] = the tile holding
+ * compacted index 256 j.
+ * One workgroup walks the counts in chunks of SCAN_CHUNK.  For large
+ * launches (chunk_sum != nullptr) k_tile_chunk_sums first sums every chunk
+ * and workgroup b scans chunk b alone, starting from the sum of the chunks
+ * before it; the grid then covers every chunk (the host's tile bound is at
+ * least T). */
+constexpr int SCAN_CHUNK = 16 * 1024;
+
+static __global__ __launch_bounds__(1024) void k_tile_chunk_sums(const unsigned int *__restrict__ cnt,
+                                                                 const int *__restrict__ M_in, int new_in_block,
+                                                                 int *__restrict__ chunk_sum) {
+    __shared__ int sh_w[16];
+    const int T = (*M_in + new_in_block + WG - 1) / WG;
+    const int base = (int)blockIdx.x * SCAN_CHUNK;
+    if (base >= T) return;   /* the whole workgroup */
+    const int b = base + (int)threadIdx.x * 16;
+    int sum = 0;
+    if (b + 16 <= T) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(cnt + b);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint4 u = q[k];
+            sum += (int)(u.x + u.y + u.z + u.w);
+        }
+    } else {
+        for (int i = 0; i < 16; i++) sum += b + i < T ? (int)cnt[b + i] : 0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63) == 0) sh_w[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < 16; k++) t += sh_w[k];
+        chunk_sum[blockIdx.x] = t;
+    }
+}
+
 static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *__restrict__ cnt, const int *__restrict__ M_in,
                                                            int new_in_block, int *__restrict__ pre, int *__restrict__ src0,
                                                            int *__restrict__ M_out, int *__restrict__ T_out,
-                                                           int *M_host) {
+                                                           int *M_host, const int *__restrict__ chunk_sum) {
     /* 16 counts per thread and round (four 16-byte loads), wave scans by
      * shuffles, one LDS exchange of the 16 wave totals */
     __shared__ int sh_w[16];
+    __shared__ int sh_carry;
     const int T = (*M_in + new_in_block + WG - 1) / WG;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int first = (int)blockIdx.x * SCAN_CHUNK;
     int carry = 0;
-    for (int base = 0; base < T; base += 16 * 1024) {
+    if (chunk_sum != nullptr && first < T) {
+        /* survivors of the chunks before this workgroup's */
+        if (w == 0) {
+            int c = 0;
+            for (int k = lane; k < (int)blockIdx.x; k += 64) c += chunk_sum[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+            if (lane == 0) sh_carry = c;
+        }
+        __syncthreads();
+        carry = sh_carry;
+    }
+    const int stride = chunk_sum != nullptr ? (int)gridDim.x * SCAN_CHUNK : SCAN_CHUNK;
+    for (int base = first; base < T; base += stride) {
         const int b = base + (int)threadIdx.x * 16;
         unsigned int v[16];
         if (b + 16 <= T) {
@@ -61,7 +113,7 @@ static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *_
             if (lane < 16) sh_w[lane] = t;
         }
         __syncthreads();
-        if (threadIdx.x == 0 && base + 16 * 1024 >= T) {
+        if (threadIdx.x == 0 && base + SCAN_CHUNK >= T) {
             /* the last round: the totals are known, and their stores (one
              * of them to host memory) overlap the prefix stores below */
             const int total = carry + sh_w[15];
@@ -95,7 +147,7 @@ static __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int *_
         carry += sh_w[15];
         __syncthreads();
     }
-    if (T == 0 && threadIdx.x == 0) {
+    if (T == 0 && threadIdx.x == 0 && blockIdx.x == 0) {
         pre[0] = 0;
         *M_out = 0;
         *T_out = 0;

@@ -179,6 +179,7 @@ struct PlainRun {
     int64_t tiles_cap = 0;              /* entries of d_cnt and of each d_src0 half */
     DevBuf<unsigned int> d_cnt;
     DevBuf<int> d_pre, d_src0, d_T;
+    DevBuf<int> d_chunk;                /* per-chunk survivor sums (multi-workgroup tile scan) */
     DevBuf<int> d_err;
     PinBuf<int> h_ring;                 /* slot count after each launch */
     std::vector<hipEvent_t> ev_ring;
@@ -272,6 +273,7 @@ struct PlainRun {
             GF_TRY(d_cnt.alloc((size_t)nt));
             GF_TRY(d_pre.alloc(2 * (size_t)pre_stride(nt)));
             GF_TRY(d_src0.alloc(2 * (size_t)nt));
+            GF_TRY(d_chunk.alloc((size_t)((nt + SCAN_CHUNK - 1) / SCAN_CHUNK)));
             tiles_cap = nt;
         }
         if (!d_T.p) GF_TRY(d_T.alloc(2));
@@ -552,9 +554,20 @@ struct PlainRun {
         t_adv.end(ev);
         /* survivors per tile -> prefix, slot count, next launch's source tiles */
         const int slot = (int)(launches % RING);
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, st, d_cnt.p, d_M.p + mcur, (int)new_in_block,
-                           d_pre.p + (par ^ 1) * pre_half, d_src0.p + (par ^ 1) * src_half, d_M.p + (mcur ^ 1),
-                           d_T.p + (par ^ 1), &h_ring.p[slot]);
+        /* more than two chunks of tiles: sum the chunks, then scan them in
+         * parallel (one workgroup walking 14 chunks took 110 us) */
+        const int nchunks = (int)((ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK);
+        const bool chunked = nchunks > 2;
+        if (chunked) {
+            if ((size_t)nchunks > d_chunk.n) { set_error("tile scan: chunk buffer smaller than the launch"); return GF_ERR_DEVICE; }
+            hipLaunchKernelGGL(k_tile_chunk_sums, dim3(nchunks), dim3(1024), 0, st, d_cnt.p, d_M.p + mcur,
+                               (int)new_in_block, d_chunk.p);
+            GF_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_tile_scan, dim3(chunked ? nchunks : 1), dim3(1024), 0, st, d_cnt.p, d_M.p + mcur,
+                           (int)new_in_block, d_pre.p + (par ^ 1) * pre_half, d_src0.p + (par ^ 1) * src_half,
+                           d_M.p + (mcur ^ 1), d_T.p + (par ^ 1), &h_ring.p[slot],
+                           chunked ? (const int *)d_chunk.p : (const int *)nullptr);
         GF_HIP(hipGetLastError());
         freqA.swap(freqB);
         idsA.swap(idsB);

@@ -213,3 +213,19 @@ def test_joint_sfs_matches_oracle(gpu, oracle):
     j = r["final_sfs"][(0, 1, 40, 30)]
     o = oracle.joint_site_frequency_spectrum(last["freq"][0], last["freq"][1], 40, 30, d.num_sites)
     assert np.array_equal(j.view(np.uint32), o.view(np.uint32))
+
+
+def test_large_launch_chunked_tile_scan(gpu, oracle):
+    """launches of more than two chunks of 16,384 tiles (about 1e7 slots:
+    1.6e10 sites at N = 1e4 hold 6.7e6 live mutations and gain 3.2e5 a
+    generation) take the chunk-sum + parallel tile scan; the run must equal
+    the oracle's bit for bit, every generation's live count included"""
+    d = S.constant(N=10000, L=1.6e10, G=12, s=S.selection_from_gamma(-2.0, 10000, 0.0), sample_every=6)
+    g = gpu.run(d)
+    oracle.set_threads(16)
+    try:
+        o = oracle.run(d)
+    finally:
+        oracle.set_threads(1)
+    assert g["samples"][0]["num_mutations"] > 6_000_000
+    assert_same(g, o, "chunked scan")
