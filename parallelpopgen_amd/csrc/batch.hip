@@ -371,9 +371,10 @@ struct BatchRun {
             for (int k = 0; k < nl; k++) h_64.p[nl + k] = nbase[k];
             GF_HIP(hipMemcpyAsync(d_base64.p + nl, h_64.p + nl, (size_t)nl * sizeof(int64_t), hipMemcpyHostToDevice, st));
             if (ntiles > 0) {
-                hipLaunchKernelGGL(k_scatter<1>, dim3(ntiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p, freqB.p,
-                                   (int64_t)freqB.n, idsB.p, d_base64.p, d_base64.p + nl, d_meta.p + o_pres,
-                                   d_meta.p + o_tile_first, nl, d_masks.p, d_toff.p);
+                hipLaunchKernelGGL(k_scatter<1>, dim3((ntiles + SCATTER_TILES - 1) / SCATTER_TILES), dim3(WG), 0, st,
+                                   freqA.p, (int64_t)cap, idsA.p, freqB.p, (int64_t)freqB.n, idsB.p, d_base64.p,
+                                   d_base64.p + nl, d_meta.p + o_pres, d_meta.p + o_tile_first, nl, d_masks.p, d_toff.p,
+                                   ntiles);
                 GF_HIP(hipGetLastError());
             }
             t_cmp.end(ce);
