@@ -33,6 +33,11 @@ Other workloads (--workload), each printing one JSON line of its own:
   surrogate  trains the surrogate ensemble on batched simulations and
              reports its predictions/s, the batched simulator's SFS/s and
              the held-out deviance against two simulation replicates
+  plain_small
+             the small single-population scenario (N = 1e4 diploids,
+             L = 2e7, neutral, 1,000 generations from equilibrium) + SFS
+             n = 100: mean of 10 runs after 3 warm-up runs, and the whole
+             run on the CPU oracle (one thread) beside it
   speed      the reference's own speed benchmark (examples/example_speed/
              run.cu:12-25: 2e5 haploid individuals, neutral, mu = 1e-9,
              1,000 generations from equilibrium, L = xL * 2e7 sites) at
@@ -67,7 +72,7 @@ def parse():
     ap.add_argument("--generations", type=int, default=5000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-generations", type=int, default=0, help="generations of the CPU sample (0: auto)")
-    ap.add_argument("--workload", choices=["plain_large", "three_population", "batched", "surrogate", "speed"],
+    ap.add_argument("--workload", choices=["plain_large", "plain_small", "three_population", "batched", "surrogate", "speed"],
                     default="plain_large")
     ap.add_argument("--sims", type=int, default=4096, help="batched: simulations in the sweep")
     ap.add_argument("--plain-sample", type=int, default=32, help="batched: simulations timed back to back")
@@ -259,6 +264,50 @@ def bench_speed(a, capi, scenarios):
     }
 
 
+def bench_plain_small(a, capi, scenarios):
+    """The small scenario: 8.4e3 live mutations, launch-bound on the GPU."""
+    d = scenarios.plain_small()
+    for _ in range(3):
+        capi.run(d, final_sfs=[(0, 100)])
+    t0 = time.perf_counter()
+    for _ in range(10):
+        r = capi.run(d, final_sfs=[(0, 100)])
+    dt = (time.perf_counter() - t0) / 10
+    mg = int(r["live"][1:].sum())
+    out = {
+        "metric": "mutation-generations/s",
+        "value": mg / dt,
+        "unit": "mutation-generations/s",
+        "n_gpus": 1,
+        "steps": 10,
+        "warmup": 3,
+        "ms_per_step": 1e3 * dt,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (simulator's own neutral-equilibrium start)",
+        "config": {
+            "workload": "plain_small: run_sim N=1e4 diploids, F=0, mu=1e-9, L=2e7, neutral, 1000 generations, "
+                        "compact 35, + SFS n=100",
+            "live_mutations_final": int(r["samples"][-1]["num_mutations"]),
+            "mutation_generations_per_step": mg,
+            "parallelism": "single GPU",
+        },
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        O.load()
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        o = O.run(d)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=int(o["live"][1:].sum()) / ct, unit="mutation-generations/s", cores=1,
+                                   kind="port", sample="the whole plain_small run on the CPU oracle, 1 thread, "
+                                                       "%.2f s" % ct)
+    return out
+
+
 def bench_three_population(a, capi, scenarios):
     """Three populations (exponential growth 2e4 -> 2e5, a bottleneck, F = 0.1,
     migration 1e-4, sine-wave selection, preserve every 100 generations),
@@ -322,6 +371,9 @@ def main():
         return
     if a.workload == "three_population":
         print(json.dumps(bench_three_population(a, capi, scenarios)))
+        return
+    if a.workload == "plain_small":
+        print(json.dumps(bench_plain_small(a, capi, scenarios)))
         return
     if a.workload == "speed":
         print(json.dumps(bench_speed(a, capi, scenarios)))

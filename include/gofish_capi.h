@@ -297,7 +297,8 @@ const char *gf_last_error(void);
 int32_t gf_set_kernel_timing(int32_t enable);
 /* Number of generations the drift kernel advances per launch at most
  * (temporal blocking depth); 1 gives one generation per launch, 0 restores
- * the default (10 for one population, 20 for several, 16 for batches). */
+ * the default (one population: 64 while at most 2e5 slots are in use, 16 up to
+ * 1.2e6, 10 beyond; 20 for several; 16 for batches). */
 int32_t gf_set_generations_per_launch(int32_t k);
 int32_t gf_device_count(int32_t *out);
 
