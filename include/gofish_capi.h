@@ -298,7 +298,8 @@ int32_t gf_set_kernel_timing(int32_t enable);
 /* Number of generations the drift kernel advances per launch at most
  * (temporal blocking depth); 1 gives one generation per launch, 0 restores
  * the default (one population: 64 while at most 2e5 slots are in use, 16 up to
- * 1.2e6, 10 beyond; 20 for several; 16 for batches). */
+ * 1.2e6, 10 beyond; several: 64 up to 2e6 slots, 20 beyond, a launch
+ * ending at each compaction generation either way; batches: 16). */
 int32_t gf_set_generations_per_launch(int32_t k);
 int32_t gf_device_count(int32_t *out);
 

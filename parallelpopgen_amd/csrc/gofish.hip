@@ -651,14 +651,18 @@ struct PlainRun {
          * (N = 1e4, 1,000 generations): 64 at 8.4e3 and 8.4e4 live
          * mutations (-41 % and -38 % against 10), 16 at 8.4e5; 10 at 2.0e6
          * (N = 1e5, theta = 2e5).  Several populations end launches at the
-         * compaction interval, 20 generations otherwise (three populations
-         * of 2e4-2e5 over 2,000 generations: 16 per launch 158 ms, 20 146 ms,
-         * 24 146 ms). */
+         * compaction interval anyway: up to 2e6 slots a launch runs to it
+         * (two populations, N = 1e4, 1,000 generations, interval 35: 20
+         * generations per launch 3.73 / 5.82 / 21.9 ms at 1.4e4 / 1.4e5 /
+         * 1.4e6 live, up to the interval 3.35 / 4.95 / 19.6 ms), 20
+         * generations beyond (three populations of 2e4-2e5 over 2,000
+         * generations: 16 per launch 158 ms, 20 146 ms, 24 146 ms). */
         const int k_user = generations_per_launch(0);
         auto launch_length = [&]() {
-            int k = 20;
+            int k;
             if (k_user > 0) k = k_user;
             else if (P == 1) k = M_bound <= 200000 ? 64 : (M_bound <= 1200000 ? 16 : 10);
+            else k = M_bound <= 2000000 ? 64 : 20;
             return std::max(1, std::min(KMAX, k));
         };
         /* one population: a lost or fixed mutation never changes again, so
