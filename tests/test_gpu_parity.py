@@ -33,7 +33,7 @@ def test_plain_small_full(gpu, oracle):
     assert g["samples"][-1]["num_mutations"] > 7000
 
 
-@pytest.mark.parametrize("kpl", [1, 5, 64])
+@pytest.mark.parametrize("kpl", [1, 5, 16, 64])
 def test_generations_per_launch_invariance(gpu, oracle, kpl):
     d = S.plain_small(G=150, compact_interval=20)
     ref = oracle.run(d)
