@@ -13,7 +13,8 @@
  * explicit fused multiply-add (fmaf) only, so the GPU library (which
  * evaluates the same operation sequence) must agree bit for bit.
  *
- * Pinning (tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle_math.py, tests/test_oracle_model.py; each pin
+ * of the model functions is also shown to fail on a planted mistake, below):
  *   philox4x32_10      Random123 known-answer vectors and vectors produced by
  *                      the reference's vendored philox.h built with g++
  *                      (oracle/Makefile -> oracle/_ref/philox_kat)
@@ -23,16 +24,33 @@
  *                      symmetry; monotonicity
  *   poiscdfinv/binomcdfinv  brute-force long-double CDF inversion with the
  *                      same 70-term truncation and mean thresholds
- *   drift update       closed forms (neutral -> identity selection, absorbing
- *                      0 and 1, haploid F=1 form) and moments over many draws
- *   MSE init           neutral closed form E[M] = 2*mu*L*Nc*H(Nc-1); the
- *                      reference's published 2,050,534 mutations
- *                      (examples/example_speed/example_speed_780M.out:3)
- *   compaction         brute force on tiny arrays (stable order, preserve)
- *   SFS                brute-force sum_k C(n,k) p^k q^(n-k) in long double;
- *                      neutral theta/i expectation over many seeds
- *   whole path         dadi-scenario regression against the reference's
- *                      examples/example_dadi/run_980_2.out
+ *   binom_draw, pois_draw   first two moments over many draws
+ *   pre_draw           Eq. 13 of the reference (go_fish_impl.cuh:105) in
+ *                      float64, and its closed forms (s = 0 -> x; F = 1 ->
+ *                      x(1+s)/(1+sx); h = 0, h = 1/2, h = 1; s clamped at
+ *                      -1); migration as a float64 weighted sum; mean and
+ *                      variance of the draw
+ *   drift_one          moments of the drawn count over 1e5 mutation IDs in
+ *                      the normal regime and both Poisson tails; 0 and 1
+ *                      absorbing
+ *   mse_lambda         a float64 restatement of go_fish_impl.cuh:20-71 with
+ *                      its quirks, over h, F and gamma grids; the neutral
+ *                      total 2 mu L Nc H(Nc-1); class counts ~ Poisson
+ *   new_count          Poisson moments over many generations
+ *   compact, keep_slot brute force on tiny arrays with present masks
+ *                      (keep rule, stable order, preserve negation)
+ *   pop_hist           numpy: round half away from zero, bin Nc folded to 0,
+ *                      (int)num_sites
+ *   sfs, joint_sfs     float64 binomial sums; total mass; marginals
+ *   whole path         the reference's published speed-test SNP counts and
+ *                      neutral theta/i over many seeds (both through the GPU
+ *                      library, which equals this oracle bit for bit);
+ *                      dadi-scenario regression against the reference's
+ *                      examples/example_dadi/run_980.out and run_980_2.out
+ *
+ * ORC_PLANT (0 in every build but the pin tests' own): compile-time planted
+ * mistakes, one per value, each a plausible slip in a model function; the
+ * pin tests build each variant and check that some pin fails on it.
  *
  * Deliberate, documented divergences from the reference (all listed in the
  * project's design notes):
@@ -64,6 +82,10 @@
 #include <functional>
 
 #include "../include/gofish_capi.h"
+
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 
 namespace orc {
 
@@ -298,7 +320,11 @@ static int binom_draw(uint32_t rbits, float mean, float var, float N) {
         }
         return f2i_sat(N - (float)poiscdfinv(u, N - mean));
     }
+#if ORC_PLANT == 15
+    return f2i_sat(round_away(normcdfinv_f(u) * var + mean));   /* planted: variance for sd */
+#else
     return f2i_sat(round_away(normcdfinv_f(u) * sqrtf(var) + mean));
+#endif
 }
 
 /* Approximate Poisson draw (reference shared.cuh:251-256, ApproxRandPois1) */
@@ -319,7 +345,11 @@ static inline int nchrom(int N, float F) {
 
 /* mse(), reference go_fish_impl.cuh:20-22 */
 static inline float mse(float i, int N, float F, float h, float s) {
+#if ORC_PLANT == 12
+    float A = ((2.0f * h) + ((1.0f - (2.0f * h)) * i)) * (1.0f + F) + (2.0f * F);   /* planted: F-term sign */
+#else
     float A = ((2.0f * h) + ((1.0f - (2.0f * h)) * i)) * (1.0f - F) + (2.0f * F);
+#endif
     float e = ((((float)(2 * N) * s) * i) * A) / (1.0f + F);
     return exp_f(e);
 }
@@ -342,6 +372,44 @@ struct GenRow {
     float mig[GF_MAX_POPULATIONS][GF_MAX_POPULATIONS];  /* [from][to] */
 };
 
+/* Migration and selection of population p (Nc > 0): the frequency after
+ * selection xs and the mean and variance of the drift draw.  Migration is
+ * the conservative weighted sum over the source populations; selection is
+ * Eq. 13 of the reference, (s x^2 + x + (F+h-hF) s x (1-x)) /
+ * (x^2 s + (F+2h-2hF) s x (1-x) + 1) with s = max(s, -1)
+ * (go_fish_impl.cuh:97-106), restated as x + s x (x + c1 (1 - x)) over
+ * 1 + s x (x + c2 (1 - x)) with fused multiply-adds. */
+static inline void pre_draw(const GenRow &t, const float *x, int p, float &xs, float &mean, float &var) {
+    float im = 0.0f;
+    for (int k = 0; k < t.P; k++) im = im + t.mig[k][p] * x[k];
+    float s = sel_eval(t.sa[p], t.sb[p], im);
+#if ORC_PLANT != 11
+    s = s > -1.0f ? s : -1.0f;
+#endif
+    float F = t.F[p], h = t.h[p];
+#if ORC_PLANT == 2
+    float c1 = (F + h) + (h * F);                         /* planted: F-term sign */
+#else
+    float c1 = (F + h) - (h * F);
+#endif
+    float c2 = (F + (2.0f * h)) - ((2.0f * h) * F);
+#if ORC_PLANT == 1
+    std::swap(c1, c2);                                    /* planted: c1 and c2 swapped */
+#endif
+    float om = 1.0f - im;
+    float sx = s * im;
+    float num = fmaf(sx, fmaf(c1, om, im), im);
+    float den = fmaf(sx, fmaf(c2, om, im), 1.0f);
+    xs = num / den;
+    float N = (float)t.Nc[p];
+    mean = xs * N;
+#if ORC_PLANT == 10
+    var = xs * mean;                                      /* planted: x for 1 - x */
+#else
+    var = (1.0f - xs) * mean;
+#endif
+}
+
 static inline void drift_one(const GenRow &t, float *x, const gf_mutid &id, int g,
                              uint32_t k0, uint32_t k1, U4 *cache, int *cache_blk) {
     float nx[GF_MAX_POPULATIONS];
@@ -349,23 +417,9 @@ static inline void drift_one(const GenRow &t, float *x, const gf_mutid &id, int 
     for (int p = 0; p < t.P; p++) {
         int Nc = t.Nc[p];
         if (Nc <= 0) { nx[p] = 0.0f; continue; }
-        float im = 0.0f;
-        for (int k = 0; k < t.P; k++) im = im + t.mig[k][p] * x[k];
-        float s = sel_eval(t.sa[p], t.sb[p], im);
-        s = s > -1.0f ? s : -1.0f;
-        float F = t.F[p], h = t.h[p];
-        float c1 = (F + h) - (h * F);
-        float c2 = (F + (2.0f * h)) - ((2.0f * h) * F);
-        float om = 1.0f - im;
-        /* Eq. 13 as x + s x (x + c1 (1 - x)) over 1 + s x (x + c2 (1 - x)),
-         * with fused multiply-adds */
-        float sx = s * im;
-        float num = fmaf(sx, fmaf(c1, om, im), im);
-        float den = fmaf(sx, fmaf(c2, om, im), 1.0f);
-        float xs = num / den;
+        float xs, mean, var;
+        pre_draw(t, x, p, xs, mean, var);
         float N = (float)Nc;
-        float mean = xs * N;
-        float var = (1.0f - xs) * mean;
         int blk = g >> 2;
         U4 r;
         if (cache && cache_blk[p] == blk) r = cache[p];
@@ -386,7 +440,11 @@ static inline void drift_one(const GenRow &t, float *x, const gf_mutid &id, int 
 static inline bool keep_slot(const GenRow &t, const float *x, const gf_mutid &id) {
     bool zero = true, one = true;
     for (int p = 0; p < t.P; p++) {
+#if ORC_PLANT == 8
+        if (true) {                                       /* planted: present mask ignored */
+#else
         if (t.present[p]) {
+#endif
             zero = zero && (x[p] <= 0.0f);
             one = one && (x[p] >= 1.0f);
         }
@@ -467,15 +525,17 @@ static void scan_chunked(const std::vector<float> &a, std::vector<float> &S) {
     }
 }
 
-/* Mutation-selection-equilibrium class counts for one population
- * (reference integrate_mse go_fish_impl.cuh:201-223,
- * initialize_mse_frequency_array :58-71). */
-static void mse_counts(const gf_pop_gen &pg, int pop, float L, uint32_t k0, uint32_t k1,
-                       std::vector<int> &counts) {
+/* Expected mutation count of every frequency class k/Nc, k = 1..Nc-1, at
+ * mutation-selection equilibrium for one population (reference
+ * integrate_mse go_fish_impl.cuh:201-223, initialize_mse_frequency_array
+ * :58-71, with the quirks kept: the upper integral of class id starts at
+ * id/Nc, j = ((Nc - id) + 1)/Nc, the integrand's exponent uses
+ * -max(s, -1)). */
+static void mse_lambda(const gf_pop_gen &pg, float L, std::vector<float> &lam) {
     int Nind = pg.N;
     float F = pg.F, h = pg.h, mu = pg.mu;
     int Nc = nchrom(Nind, F);
-    counts.assign(Nc > 1 ? Nc - 1 : 0, 0);
+    lam.assign(Nc > 1 ? Nc - 1 : 0, 0.0f);
     if (Nc <= 1) return;
     /* the integral is only needed when some class has s != 0 */
     bool neutral = (pg.sel_slope == 0.0f && pg.sel_intercept == 0.0f);
@@ -487,31 +547,56 @@ static void mse_counts(const gf_pop_gen &pg, int pop, float L, uint32_t k0, uint
             float a = 1.0f - (float)id * step;
             float s1 = sel_eval(pg.sel_slope, pg.sel_intercept, a);
             s1 = s1 > -1.0f ? s1 : -1.0f;
-            float f1 = mse(a, Nind, F, h, -1.0f * s1);
             float a2 = a - step;
             float s2 = sel_eval(pg.sel_slope, pg.sel_intercept, a2);
             s2 = s2 > -1.0f ? s2 : -1.0f;
+#if ORC_PLANT == 5
+            float f1 = mse(a, Nind, F, h, s1), f2 = mse(a2, Nind, F, h, s2);   /* planted: integrand sign */
+#else
+            float f1 = mse(a, Nind, F, h, -1.0f * s1);
             float f2 = mse(a2, Nind, F, h, -1.0f * s2);
+#endif
             area[id] = (step * (f1 + f2)) / 2.0f;
         }
         std::vector<float> S;
         scan_chunked(area, S);
-        I.resize(Nc);
+        I.resize(Nc + 1);
         for (int id = 0; id < Nc; id++) I[id] = S[Nc - 1 - id];
+        I[Nc] = 0.0f;
     }
     float total = neutral ? 0.0f : I[0];
     float twomuL = (2.0f * mu) * L;
-    float Nn = L * (float)Nc;
     for (int id = 0; id < Nc - 1; id++) {
         float i = ((float)id + 1.0f) / (float)Nc;
+#if ORC_PLANT == 3
+        float j = 1.0f - i;                               /* planted: j without the quirk */
+#else
         float j = ((float)(Nc - id) + 1.0f) / (float)Nc;
+#endif
         float s = sel_eval(pg.sel_slope, pg.sel_intercept, i);
-        float lambda;
-        if (s == 0.0f) lambda = twomuL / i;
-        else lambda = (twomuL * (mse(i, Nind, F, h, s) * I[id])) / ((total * i) * j);
+#if ORC_PLANT == 4
+        const float Iu = neutral ? 0.0f : I[id + 1];      /* planted: integral from (id+1)/Nc */
+#else
+        const float Iu = neutral ? 0.0f : I[id];
+#endif
+        if (s == 0.0f) lam[id] = twomuL / i;
+        else lam[id] = (twomuL * (mse(i, Nind, F, h, s) * Iu)) / ((total * i) * j);
+    }
+}
+
+/* Mutation-selection-equilibrium class counts for one population: a
+ * Poisson draw of each class's expected count with counter {0, id, pop, 0}
+ * (reference initialize_mse_frequency_array go_fish_impl.cuh:58-71). */
+static void mse_counts(const gf_pop_gen &pg, int pop, float L, uint32_t k0, uint32_t k1,
+                       std::vector<int> &counts) {
+    std::vector<float> lam;
+    mse_lambda(pg, L, lam);
+    counts.assign(lam.size(), 0);
+    const float Nn = L * (float)nchrom(pg.N, pg.F);
+    for (size_t id = 0; id < lam.size(); id++) {
         U4 c = { 0u, (uint32_t)id, (uint32_t)pop, 0u };
         U4 r = philox10(k0, k1, c);
-        int n = pois_draw(r.x, lambda, lambda, Nn);
+        int n = pois_draw(r.x, lam[id], lam[id], Nn);
         counts[id] = n > 0 ? n : 0;
     }
 }
@@ -555,7 +640,7 @@ static void compact(Sim &S, const GenRow &row, bool preserve) {
             for (int p = 0; p < P; p++) S.x[w * P + p] = S.x[i * P + p];
             S.id[w] = S.id[i];
         }
-        if (preserve) {
+        if (preserve && ORC_PLANT != 9) {
             int z = S.id[w].origin_threadID;
             S.id[w].origin_threadID = -(z < 0 ? -z : z);
         }
@@ -744,11 +829,19 @@ static void parallel_for(size_t n, const std::function<void(size_t, size_t, int)
 static void pop_hist(const float *freq, int M, int Nc, float num_sites, std::vector<uint32_t> &h) {
     h.assign(Nc, 0);
     for (int i = 0; i < M; i++) {
+#if ORC_PLANT == 6
+        int idx = f2i_sat(rintf((float)Nc * freq[i]));    /* planted: ties to even */
+#else
         int idx = f2i_sat(round_away((float)Nc * freq[i]));
-        if (idx == Nc) idx = 0;
+#endif
+        if (idx == Nc && ORC_PLANT != 7) idx = 0;        /* ORC_PLANT 7: bin Nc not folded */
         if (idx >= 0 && idx < Nc) h[idx]++;
     }
+#if ORC_PLANT == 13
+    int ns = f2i_sat(round_away(num_sites));              /* planted: num_sites rounded */
+#else
     int ns = f2i_sat(truncf(num_sites));
+#endif
     h[0] += (uint32_t)(ns - M);
 }
 
@@ -832,6 +925,16 @@ static void joint_sfs(const float *fa, const float *fb, int M, int na, int nb, f
     for (int k = 0; k < B; k++) out[k] = (float)((double)acc[k] * sc + (k == 0 ? (double)num_sites - (double)M : 0.0));
 }
 
+static void fill_test_row(GenRow &row, int32_t P, const int32_t *Nc, const float *F, const float *h, const float *sa,
+                          const float *sb, const float *mig) {
+    row.P = P;
+    for (int p = 0; p < P; p++) {
+        row.Nc[p] = Nc[p]; row.present[p] = Nc[p] > 0; row.F[p] = F[p]; row.h[p] = h[p];
+        row.sa[p] = sa[p]; row.sb[p] = sb[p];
+        for (int k = 0; k < P; k++) row.mig[k][p] = mig[k * P + p];
+    }
+}
+
 } /* namespace orc */
 
 /* ===================== C API (test hooks) ===================== */
@@ -871,14 +974,53 @@ int32_t orc_drift_step(int32_t P, int32_t M, float *x /*[M][P]*/, const gf_mutid
                        int32_t seed1, int32_t seed2, const int32_t *Nc, const float *F, const float *h,
                        const float *sa, const float *sb, const float *mig /*[P][P] from,to*/) {
     if (P < 1 || P > GF_MAX_POPULATIONS) return GF_ERR_ARG;
-    GenRow row; row.P = P;
-    for (int p = 0; p < P; p++) {
-        row.Nc[p] = Nc[p]; row.present[p] = Nc[p] > 0; row.F[p] = F[p]; row.h[p] = h[p];
-        row.sa[p] = sa[p]; row.sb[p] = sb[p];
-        for (int k = 0; k < P; k++) row.mig[k][p] = mig[k * P + p];
-    }
+    GenRow row;
+    fill_test_row(row, P, Nc, F, h, sa, sb, mig);
     for (int i = 0; i < M; i++) drift_one(row, &x[(size_t)i * P], ids[i], g, (uint32_t)seed1, (uint32_t)seed2, nullptr, nullptr);
     return 0;
+}
+
+/* migration + selection of M mutations x[M][P] (no draw): the frequency
+ * after selection and the mean and variance of the drift draw, [M][P] each;
+ * populations with Nc <= 0 get 0 */
+int32_t orc_pre_draw(int32_t P, int32_t M, const float *x, const int32_t *Nc, const float *F, const float *h,
+                     const float *sa, const float *sb, const float *mig, float *xs, float *mean, float *var) {
+    if (P < 1 || P > GF_MAX_POPULATIONS || M < 0) return GF_ERR_ARG;
+    GenRow row;
+    fill_test_row(row, P, Nc, F, h, sa, sb, mig);
+    for (int i = 0; i < M; i++)
+        for (int p = 0; p < P; p++) {
+            size_t o = (size_t)i * P + p;
+            xs[o] = mean[o] = var[o] = 0.0f;
+            if (Nc[p] > 0) pre_draw(row, &x[(size_t)i * P], p, xs[o], mean[o], var[o]);
+        }
+    return 0;
+}
+
+/* expected equilibrium count of each frequency class (Nc - 1 of them) */
+int32_t orc_mse_lambda(const gf_pop_gen *pg, float L, float *out, int32_t cap) {
+    std::vector<float> lam;
+    mse_lambda(*pg, L, lam);
+    for (size_t i = 0; i < lam.size() && (int)i < cap; i++) out[i] = lam[i];
+    return (int32_t)lam.size();
+}
+
+/* one compaction of M slots x[M][P] / ids (in place): keep rule over the
+ * populations with present[p] != 0, stable order, preserve negation;
+ * returns the number kept */
+int32_t orc_compact_step(int32_t P, int32_t M, float *x, gf_mutid *ids, const int32_t *present, int32_t preserve) {
+    if (P < 1 || P > GF_MAX_POPULATIONS || M < 0) return GF_ERR_ARG;
+    Sim S;
+    S.P = P;
+    S.x.assign(x, x + (size_t)M * P);
+    S.id.assign(ids, ids + M);
+    GenRow row;
+    row.P = P;
+    for (int p = 0; p < P; p++) row.present[p] = present[p] != 0;
+    compact(S, row, preserve != 0);
+    memcpy(x, S.x.data(), S.x.size() * sizeof(float));
+    memcpy(ids, S.id.data(), S.id.size() * sizeof(gf_mutid));
+    return (int32_t)S.M();
 }
 
 int32_t orc_mse_counts(const gf_pop_gen *pg, int32_t pop, float L, int32_t seed1, int32_t seed2, int32_t *out, int32_t cap) {

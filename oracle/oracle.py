@@ -31,6 +31,53 @@ def _cpu_has_fma():
     return False
 
 
+_I32, _I64, _F32, _VP = C.c_int32, C.c_int64, C.c_float, C.c_void_p
+_SIGS = {
+    "orc_set_threads": (_I32, [_I32]),
+    "orc_philox": (None, [C.c_uint32, C.c_uint32, _VP, _VP]),
+    "orc_u01_array": (None, [_VP, _VP, _I64]),
+    "orc_exp_array": (None, [_VP, _VP, _I64]),
+    "orc_log_array": (None, [_VP, _VP, _I64]),
+    "orc_normcdfinv_array": (None, [_VP, _VP, _I64]),
+    "orc_poiscdfinv_array": (None, [_VP, _VP, _VP, _I64]),
+    "orc_binomcdfinv_array": (None, [_VP, _VP, _VP, _VP, _VP, _I64]),
+    "orc_binom_draw_array": (None, [_VP, _VP, _VP, _VP, _VP, _I64]),
+    "orc_pois_draw_array": (None, [_VP, _VP, _VP, _VP, _I64]),
+    "orc_round_away_array": (None, [_VP, _VP, _I64]),
+    "orc_f2i_array": (None, [_VP, _VP, _I64]),
+    "orc_drift_step": (_I32, [_I32, _I32, _VP, _VP, _I32, _I32, _I32, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "orc_pre_draw": (_I32, [_I32, _I32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "orc_mse_lambda": (_I32, [_VP, _F32, _VP, _I32]),
+    "orc_compact_step": (_I32, [_I32, _I32, _VP, _VP, _VP, _I32]),
+    "orc_mse_counts": (_I32, [_VP, _I32, _F32, _I32, _I32, _VP, _I32]),
+    "orc_new_count": (_I32, [_I32, _F32, _F32, _I32, _I32, _I32, _I32]),
+    "orc_run": (_I32, [_VP, C.POINTER(_VP)]),
+    "orc_result_num_samples": (_I32, [_VP]),
+    "orc_result_sample_info": (_I32, [_VP, _I32, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32),
+                                      C.POINTER(C.c_uint8)]),
+    "orc_result_sample_freq": (_I32, [_VP, _I32, _VP]),
+    "orc_result_num_ids": (_I32, [_VP]),
+    "orc_result_ids": (_I32, [_VP, _VP]),
+    "orc_result_live_counts": (_I32, [_VP, _VP]),
+    "orc_result_free": (None, [_VP]),
+    "orc_population_histogram": (_I32, [_VP, _I32, _I32, _F32, _VP]),
+    "orc_site_frequency_spectrum": (_I32, [_VP, _I32, _I32, _F32, _VP]),
+    "orc_joint_site_frequency_spectrum": (_I32, [_VP, _VP, _I32, _I32, _I32, _F32, _VP]),
+}
+
+
+def bind(path):
+    """Opens an oracle build at `path` with every entry point's signature
+    set (load() uses it for the library's own build; the pin tests for
+    builds with a planted mistake)."""
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
 def load():
     """Opens the oracle library: the -mfma build where the CPU has FMA (same
     results, fmaf inline), else the portable one."""
@@ -39,43 +86,8 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH) or not os.path.exists(LIB_FMA_PATH):
         build()
-    lib = C.CDLL(LIB_FMA_PATH if _cpu_has_fma() else LIB_PATH)
-    vp = C.c_void_p
-    i32, i64, f32 = C.c_int32, C.c_int64, C.c_float
-    sig = {
-        "orc_set_threads": (i32, [i32]),
-        "orc_philox": (None, [C.c_uint32, C.c_uint32, vp, vp]),
-        "orc_u01_array": (None, [vp, vp, i64]),
-        "orc_exp_array": (None, [vp, vp, i64]),
-        "orc_log_array": (None, [vp, vp, i64]),
-        "orc_normcdfinv_array": (None, [vp, vp, i64]),
-        "orc_poiscdfinv_array": (None, [vp, vp, vp, i64]),
-        "orc_binomcdfinv_array": (None, [vp, vp, vp, vp, vp, i64]),
-        "orc_binom_draw_array": (None, [vp, vp, vp, vp, vp, i64]),
-        "orc_pois_draw_array": (None, [vp, vp, vp, vp, i64]),
-        "orc_round_away_array": (None, [vp, vp, i64]),
-        "orc_f2i_array": (None, [vp, vp, i64]),
-        "orc_drift_step": (i32, [i32, i32, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
-        "orc_mse_counts": (i32, [vp, i32, f32, i32, i32, vp, i32]),
-        "orc_new_count": (i32, [i32, f32, f32, i32, i32, i32, i32]),
-        "orc_run": (i32, [vp, C.POINTER(vp)]),
-        "orc_result_num_samples": (i32, [vp]),
-        "orc_result_sample_info": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint8)]),
-        "orc_result_sample_freq": (i32, [vp, i32, vp]),
-        "orc_result_num_ids": (i32, [vp]),
-        "orc_result_ids": (i32, [vp, vp]),
-        "orc_result_live_counts": (i32, [vp, vp]),
-        "orc_result_free": (None, [vp]),
-        "orc_population_histogram": (i32, [vp, i32, i32, f32, vp]),
-        "orc_site_frequency_spectrum": (i32, [vp, i32, i32, f32, vp]),
-        "orc_joint_site_frequency_spectrum": (i32, [vp, vp, i32, i32, i32, f32, vp]),
-    }
-    for name, (res, args) in sig.items():
-        fn = getattr(lib, name)
-        fn.restype = res
-        fn.argtypes = args
-    _lib = lib
-    return lib
+    _lib = bind(LIB_FMA_PATH if _cpu_has_fma() else LIB_PATH)
+    return _lib
 
 
 def _p(a):
@@ -157,29 +169,70 @@ def pois_draw(bits, mean, N):
     return out
 
 
-def drift_step(x, ids, g, seed1, seed2, Nc, F, h, sa, sb, mig):
+def drift_step(x, ids, g, seed1, seed2, Nc, F, h, sa, sb, mig, lib=None):
     """One generation for M mutations x[M][P] (modified copy returned)."""
+    lib = lib or load()
     x = np.ascontiguousarray(x, np.float32).copy()
     M, P = x.shape
     ids = np.ascontiguousarray(ids)
-    a = lambda v, t: np.ascontiguousarray(v, t)
-    rc = load().orc_drift_step(P, M, _p(x), _p(ids), int(g), _i32(seed1), _i32(seed2),
+    a = lambda v, t: np.ascontiguousarray(np.broadcast_to(np.asarray(v, t), (P,)))
+    mig = np.ascontiguousarray(np.broadcast_to(mig, (P, P)), np.float32)
+    rc = lib.orc_drift_step(P, M, _p(x), _p(ids), int(g), _i32(seed1), _i32(seed2),
                                _p(a(Nc, np.int32)), _p(a(F, np.float32)), _p(a(h, np.float32)),
-                               _p(a(sa, np.float32)), _p(a(sb, np.float32)), _p(a(mig, np.float32)))
+                               _p(a(sa, np.float32)), _p(a(sb, np.float32)), _p(mig))
     assert rc == 0
     return x
 
 
-def new_count(Nc, mu, L, seed1, seed2, g, pop):
-    return load().orc_new_count(int(Nc), float(mu), float(L), _i32(seed1), _i32(seed2), int(g), int(pop))
+def new_count(Nc, mu, L, seed1, seed2, g, pop, lib=None):
+    return (lib or load()).orc_new_count(int(Nc), float(mu), float(L), _i32(seed1), _i32(seed2), int(g), int(pop))
 
 
-def mse_counts(pg_row, pop, L, seed1, seed2):
+def pre_draw(x, Nc, F, h, sa, sb, mig, lib=None):
+    """Migration + selection of M mutations x[M][P] without the draw:
+    (xs, mean, var), each [M][P]."""
+    lib = lib or load()
+    x = np.ascontiguousarray(x, np.float32)
+    M, P = x.shape
+    a = lambda v, t: np.ascontiguousarray(np.broadcast_to(np.asarray(v, t), (P,)))
+    xs, mean, var = (np.empty((M, P), np.float32) for _ in range(3))
+    mig = np.ascontiguousarray(np.broadcast_to(mig, (P, P)), np.float32)
+    rc = lib.orc_pre_draw(P, M, _p(x), _p(a(Nc, np.int32)), _p(a(F, np.float32)), _p(a(h, np.float32)),
+                          _p(a(sa, np.float32)), _p(a(sb, np.float32)), _p(mig), _p(xs), _p(mean), _p(var))
+    assert rc == 0
+    return xs, mean, var
+
+
+def mse_lambda(pg_row, L, lib=None):
+    """Expected equilibrium count of classes 1..Nc-1 (pg_row: a record of
+    POP_GEN_DTYPE)."""
+    lib = lib or load()
+    rec = np.ascontiguousarray(np.array(pg_row).reshape(1))
+    cap = 2 * int(rec["N"][0]) + 2
+    out = np.zeros(cap, np.float32)
+    n = lib.orc_mse_lambda(_p(rec), float(L), _p(out), cap)
+    return out[:n]
+
+
+def compact_step(x, ids, present, preserve, lib=None):
+    """One compaction of slots x[M][P] / ids: (x, ids) kept, in order."""
+    lib = lib or load()
+    x = np.ascontiguousarray(x, np.float32).copy()
+    ids = np.ascontiguousarray(ids).copy()
+    M, P = x.shape
+    pr = np.ascontiguousarray(present, np.int32)
+    n = lib.orc_compact_step(P, M, _p(x), _p(ids), _p(pr), int(bool(preserve)))
+    assert n >= 0
+    return x[:n], ids[:n]
+
+
+def mse_counts(pg_row, pop, L, seed1, seed2, lib=None):
     """pg_row: numpy record of POP_GEN_DTYPE."""
+    lib = lib or load()
     rec = np.ascontiguousarray(np.array(pg_row).reshape(1))
     Nc = int(2 * int(rec["N"][0]) / (1 + float(rec["F"][0])))
     out = np.zeros(max(Nc, 1), np.int32)
-    n = load().orc_mse_counts(_p(rec), int(pop), float(L), _i32(seed1), _i32(seed2), _p(out), len(out))
+    n = lib.orc_mse_counts(_p(rec), int(pop), float(L), _i32(seed1), _i32(seed2), _p(out), len(out))
     return out[:n]
 
 
@@ -202,10 +255,10 @@ def run(desc):
         lib.orc_result_free(h)
 
 
-def population_frequency_histogram(freq_row, Nc, num_sites):
+def population_frequency_histogram(freq_row, Nc, num_sites, lib=None):
     f = np.ascontiguousarray(freq_row, np.float32)
     out = np.empty(int(Nc), np.float32)
-    assert load().orc_population_histogram(_p(f), len(f), int(Nc), float(num_sites), _p(out)) == 0
+    assert (lib or load()).orc_population_histogram(_p(f), len(f), int(Nc), float(num_sites), _p(out)) == 0
     return out
 
 
