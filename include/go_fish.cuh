@@ -420,6 +420,47 @@ struct selection_hooks {
     }
 };
 
+/* The model tables of a run (the tables of gf_sim_desc): one gf_pop_gen
+ * row per (generation, population), the P x P migration matrix of every
+ * generation ([from][to]) and the preserve / sample flags. */
+struct sim_tables {
+    std::vector<gf_pop_gen> params;
+    std::vector<float> mig;
+    std::vector<uint8_t> pres, samp;
+};
+
+/* Evaluates the host-callable model functors at generations start ..
+ * start + G, as the reference evaluates them on the host every generation
+ * (go_fish_impl.cuh:647-661, :454-467, :705-712), and lowers selection to
+ * its per-(population, generation) linear form where it has one. */
+template <typename Functor_mutation, typename Functor_demography, typename Functor_migration, typename Functor_selection,
+          typename Functor_inbreeding, typename Functor_dominance, typename Functor_preserve, typename Functor_timesample>
+inline void build_tables(sim_tables &T, int P, int G, int start, const Functor_mutation &mu_rate,
+                         const Functor_demography &demography, const Functor_migration &mig_prop,
+                         const Functor_selection &sel_coeff, const Functor_inbreeding &FI, const Functor_dominance &dominance,
+                         const Functor_preserve &preserve_mutations, const Functor_timesample &take_sample) {
+    T.params.assign((size_t)(G + 1) * P, gf_pop_gen());
+    T.mig.assign((size_t)(G + 1) * P * P, 0.0f);
+    T.pres.assign(G + 1, 0);
+    T.samp.assign(G + 1, 0);
+    for (int t = 0; t <= G; t++) {
+        const int g = start + t;
+        for (int p = 0; p < P; p++) {
+            gf_pop_gen &r = T.params[(size_t)t * P + p];
+            r.N = demography(p, g);
+            r.F = FI(p, g);
+            r.h = dominance(p, g);
+            r.mu = mu_rate(p, g);
+            r.sel_slope = 0.0f;
+            r.sel_intercept = 0.0f;
+            lower_selection(sel_coeff, p, g, r.sel_slope, r.sel_intercept);
+            for (int k = 0; k < P; k++) T.mig[(size_t)t * P * P + (size_t)k * P + p] = mig_prop(k, p, g);
+        }
+        T.pres[t] = preserve_mutations(g) ? 1 : 0;
+        T.samp[t] = take_sample(g) ? 1 : 0;
+    }
+}
+
 } /* namespace go_fish_details */
 
 namespace Sim_Model {
@@ -447,14 +488,6 @@ inline void run_sim(allele_trajectories &all_results, const Functor_mutation mu_
                     const Functor_dominance dominance, const Functor_preserve preserve_mutations,
                     const Functor_timesample take_sample, const allele_trajectories &prev_sim) {
     allele_trajectories::sim_constants &in = all_results.sim_input_constants;
-    /* device selection (reference set_cuda_device, shared.cuh:27-37) */
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { std::fprintf(stderr, "run_sim error: no HIP device\n"); std::exit(1); }
-    if (in.device >= 0 && in.device < ndev) (void)hipSetDevice(in.device);
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    in.device = dev;
-
     const int P = in.num_populations;
     int start = 0;
     int mode = GF_INIT_MSE;
@@ -486,23 +519,9 @@ inline void run_sim(allele_trajectories &all_results, const Functor_mutation mu_
         std::fprintf(stderr, "run_sim error: num_populations %d outside [1, %d]\n", P, GF_MAX_POPULATIONS);
         std::exit(1);
     }
-    std::vector<gf_pop_gen> params((size_t)(G + 1) * P);
-    std::vector<float> mig((size_t)(G + 1) * P * P);
-    std::vector<uint8_t> pres(G + 1), samp(G + 1);
-    for (int t = 0; t <= G; t++) {
-        const int g = start + t;
-        for (int p = 0; p < P; p++) {
-            gf_pop_gen &r = params[(size_t)t * P + p];
-            r.N = demography(p, g);
-            r.F = FI(p, g);
-            r.h = dominance(p, g);
-            r.mu = mu_rate(p, g);
-            go_fish_details::lower_selection(sel_coeff, p, g, r.sel_slope, r.sel_intercept);
-            for (int k = 0; k < P; k++) mig[(size_t)t * P * P + (size_t)k * P + p] = mig_prop(k, p, g);
-        }
-        pres[t] = preserve_mutations(g) ? 1 : 0;
-        samp[t] = take_sample(g) ? 1 : 0;
-    }
+    go_fish_details::sim_tables T;
+    go_fish_details::build_tables(T, P, G, start, mu_rate, demography, mig_prop, sel_coeff, FI, dominance,
+                                  preserve_mutations, take_sample);
     std::vector<uint8_t> ext(P, 0);
     gf_sim_desc d;
     std::memset(&d, 0, sizeof d);
@@ -521,10 +540,23 @@ inline void run_sim(allele_trajectories &all_results, const Functor_mutation mu_
         for (int p = 0; p < P; p++) ext[p] = src->extinct[p] ? 1 : 0;
         d.init_extinct = ext.data();
     }
-    d.params = params.data();
-    d.migration = mig.data();
-    d.preserve = pres.data();
-    d.sample = samp.data();
+    d.params = T.params.data();
+    d.migration = T.mig.data();
+    d.preserve = T.pres.data();
+    d.sample = T.samp.data();
+    /* the reference's per-generation parameter checks, for the whole run,
+     * before any device is touched */
+    if (gf_check_sim(&d) != GF_OK) {
+        std::fprintf(stderr, "%s\n", gf_last_error());
+        std::exit(1);
+    }
+    /* device selection (reference set_cuda_device, shared.cuh:27-37) */
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { std::fprintf(stderr, "run_sim error: no HIP device\n"); std::exit(1); }
+    if (in.device >= 0 && in.device < ndev) (void)hipSetDevice(in.device);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    in.device = dev;
     d.device = dev;
     gf_result *res = NULL;
     int32_t rc;

@@ -112,6 +112,14 @@ typedef struct gf_timing {
  * :137-176, scatter_arrays go_fish_impl.cu:147-189). */
 int32_t gf_run(const gf_sim_desc *desc, gf_result **out);
 
+/* The parameter checks gf_run makes, for every generation of `desc`, on
+ * the host only (no device is touched): GF_OK, or GF_ERR_ARG / GF_ERR_PARAM
+ * with the reference's message in gf_last_error() (replaces the
+ * per-generation check_sim_parameters, go_fish_impl.cuh:431-452, called at
+ * :596 and :647; run_sim calls this before it selects a device, so a bad
+ * model fails the same way with or without one). */
+int32_t gf_check_sim(const gf_sim_desc *desc);
+
 /* Selection evaluated by code compiled in the caller's program.  The
  * reference takes any __device__ float(int population, int generation,
  * float freq) selection functor and compiles migration_selection_drift
@@ -203,7 +211,7 @@ int32_t gf_result_joint_site_frequency_spectrum(const gf_result *result, int32_t
 
 /* ----- batched simulation (no reference counterpart: replaces the
  * back-to-back run_sim + site_frequency_spectrum loop of
- * examples/example_dadi/run.cu:139-150) ----- */
+ * examples/example_dadi/run.cu:83-94) ----- */
 typedef struct gf_batch_sim {
     int32_t N;                 /* individuals at generation 0 (equilibrium start) */
     int32_t N1;                /* individuals from generation 1 on (<= 0: N); one instantaneous size change */

@@ -59,10 +59,9 @@
  *   - the new-mutation count uses Philox counter {0, g, pop, 1} instead of
  *     {1, g, pop, 0} (the latter collides with ID-keyed drift counters);
  *   - the inverse normal CDF (CUDA's normcdfinvf in the reference, a closed
- *     library) is a division-free polynomial approximation in the variable
- *     w = -log2(4 s (1 - s)) of Giles (2011), coefficients fitted by
- *     parallelpopgen_amd/csrc/fit_normcdfinv.py; u is clamped to at most
- *     1 - 2^-24 (u == 1.0f would give +inf);
+ *     library) is a segmented cubic table over the float bits of
+ *     min(u, 1 - u), fitted by parallelpopgen_amd/csrc/fit_normcdfinv.py;
+ *     u is clamped to at most 1 - 2^-24 (u == 1.0f would give +inf);
  *   - a population that is not drifted in a generation (not yet arisen,
  *     extinct) gets frequency 0 instead of NaN/stale memory;
  *   - the MSE integral uses a fixed-order two-level float scan
@@ -183,42 +182,27 @@ static float pow_f(float a, float b) {
 }
 
 /* Inverse standard normal CDF, p in [2^-33, 1] (u01's range), clamped to
- * p <= 1 - 2^-24.  With s = min(p, 1 - p) and w = -log2(4 s (1 - s)) (the
- * change of variable of Giles, "Approximating the erfinv function", GPU
- * Computing Gems Jade Edition, 2011), ndtri(p) = (p - 1/2) R, R a polynomial
- * in w - 4 for w < 8 and in sqrt(w) - 4.2031 beyond.  log2 of
- * v = m 2^e (m in [sqrt(1/2), sqrt(2))) is e + f L(f), f = m - 1.  All
- * polynomials by Horner's rule with fused multiply-adds. */
-static const float NCDF_LOG2[10] = {
-    0x1.715476p+0f, -0x1.71547p-1f, 0x1.ec70acp-2f, -0x1.715a6ep-2f, 0x1.2779eep-2f,
-    -0x1.eab71ep-3f, 0x1.a399dep-3f, -0x1.880824p-3f, 0x1.7a19cap-3f, -0x1.b74d82p-4f};
-static const float NCDF_CENTRAL[10] = {
-    0x1.1be4ccp+2f, 0x1.ea0ccp-2f, -0x1.c6ba7p-8f, -0x1.f793d2p-11f, 0x1.1d1adcp-13f,
-    -0x1.2aad7ap-18f, -0x1.b3f612p-21f, 0x1.bf18f8p-24f, 0x1.ff977ep-31f, -0x1.a7008p-31f};
-static const float NCDF_TAIL[9] = {
-    0x1.2dd49ap+3f, 0x1.2fa2d4p+1f, 0x1.a6a788p-8f, -0x1.b5be06p-9f, 0x1.76b48ep-10f,
-    -0x1.b1259cp-11f, 0x1.fbb8cap-12f, -0x1.3a9bp-13f, 0x1.6a392ep-17f};
-
-static float poly_fma(const float *c, int n, float x) {
-    float acc = c[n - 1];
-    for (int k = n - 2; k >= 0; k--) acc = fmaf(acc, x, c[k]);
-    return acc;
-}
+ * p <= 1 - 2^-24.  With s = min(p, 1 - p) in [2^-33, 1/2], the float bits
+ * of s select a segment of the table kNcdfSeg (biased exponent and top 5
+ * mantissa bits: 16 segments per binade, logarithmically spaced towards the
+ * tail; the last segment is s = 1/2), and the low 19 mantissa bits, read
+ * as an integer t, are the argument of that segment's cubic
+ * c0 + t (c1 + t (c2 + t c3)) ~ ndtri(s), by Horner's rule with fused
+ * multiply-adds; the result is r below 1/2 and -r above.  The table is the
+ * generated header the library also uses (it defines the function, as
+ * Philox's constants define the generator); it is pinned here against
+ * scipy.special.ndtri. */
+#include "../parallelpopgen_amd/csrc/gf_ncdf_table.h"
 
 static float normcdfinv_f(float p) {
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
     const float s = q < 0.0f ? p : 1.0f - p;
-    const float v = (4.0f * s) * (1.0f - s);
-    const uint32_t b = f2bits(v);
-    int e = (int)(b >> 23) - 127;
-    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
-    if (m > 1.41421356f) { m = 0.5f * m; e += 1; }
-    const float f = m - 1.0f;
-    const float w = -(float)e - f * poly_fma(NCDF_LOG2, 10, f);
-    const float r = w < 8.0f ? poly_fma(NCDF_CENTRAL, 10, w - 4.0f)
-                             : poly_fma(NCDF_TAIL, 9, sqrtf(w) - 0x1.0cff86p+2f);
-    return q * r;
+    const uint32_t b = f2bits(s);
+    const float *c = kNcdfSeg + 4 * ((b >> GF_NCDF_T_BITS) - GF_NCDF_BASE);
+    const float t = (float)(b & ((1u << GF_NCDF_T_BITS) - 1u));
+    const float r = fmaf(fmaf(fmaf(c[3], t, c[2]), t, c[1]), t, c[0]);
+    return q < 0.0f ? r : -r;
 }
 
 /* round half away from zero (CUDA round(), used at shared.cuh:255,266) */

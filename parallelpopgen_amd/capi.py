@@ -76,6 +76,7 @@ SIGNATURES = {
     "gf_result_live_counts": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "gf_result_timing": (C.c_int32, [C.c_void_p, C.POINTER(gf_timing)]),
     "gf_result_free": (None, [C.c_void_p]),
+    "gf_check_sim": (C.c_int32, [C.POINTER(gf_sim_desc)]),
     "gf_population_histogram": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_batch_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -228,6 +229,12 @@ def check_any(rc, lib, prefix):
     if rc != 0:
         msg = lib.gf_last_error().decode() if prefix == "gf_" else "oracle error"
         raise GoFishError(rc, msg)
+
+
+def check_sim(desc: SimDesc):
+    """gf_check_sim: the run's parameter checks on the host (no device);
+    raises GoFishError with the reference's message on a bad model."""
+    check(load().gf_check_sim(C.byref(desc.desc)))
 
 
 def run(desc: SimDesc, timing=False, final_sfs=(), final_joint_sfs=()):

@@ -3,7 +3,7 @@
  * population simulations advanced together, each launch covering every
  * live simulation (no reference counterpart; it replaces the back-to-back
  * run_sim + site_frequency_spectrum loop of examples/example_dadi/run.cu:
- * 139-150, which recreates streams and buffers and synchronises the host
+ * 83-94, which recreates streams and buffers and synchronises the host
  * several times per simulation).
  *
  * All simulations share one structure-of-arrays buffer; simulation s owns

@@ -135,11 +135,12 @@ struct FunctorSel {
  * `refresh` is set (at a block boundary, or in the first generation a slot
  * is drifted in this launch), also for a population that is empty in this
  * generation, so that it is current if the population is founded later in
- * the same block. */
+ * the same block.  ncdf is the workgroup's LDS copy of the inverse normal's
+ * segment table (load_ncdf_table). */
 template <int P, typename Row, typename Sel>
 __device__ __forceinline__ void generation_update(float *x, const Row &R, int g, bool refresh, u32x4 *rc,
                                                   uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop,
-                                                  const Sel &sel) {
+                                                  const Sel &sel, const NcdfSeg *ncdf) {
     GF_STRICT_FP
     float nx[P];
 #pragma unroll
@@ -147,9 +148,13 @@ __device__ __forceinline__ void generation_update(float *x, const Row &R, int g,
         const int Nc = R.Nc[p];
         if (refresh) rc[p] = drift_block(k0, k1, tid, ogen, opop, p, g);
         if (Nc <= 0) { nx[p] = 0.0f; continue; }
-        float im = 0.0f;
+        /* one population: the migration weight is 1 (conservative
+         * migration), and 0 + 1 * x == x */
+        float im = P == 1 ? x[0] : 0.0f;
+        if (P > 1) {
 #pragma unroll
-        for (int k = 0; k < P; k++) im = im + R.mig[k][p] * x[k];
+            for (int k = 0; k < P; k++) im = im + R.mig[k][p] * x[k];
+        }
         float xs;
         if (R.neutral_mask & (1 << p)) {
             xs = im;   /* s == 0: Eq. 13 reduces exactly to x */
@@ -166,12 +171,19 @@ __device__ __forceinline__ void generation_update(float *x, const Row &R, int g,
         const float N = R.fN[p];
         const float mean = xs * N;
         const float var = (1.0f - xs) * mean;
-        int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N);
+        int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N, ncdf);
         j = j < 0 ? 0 : (j > Nc ? Nc : j);
         nx[p] = (float)((double)(float)j * R.invN[p]);   /* == (float)j / N */
     }
 #pragma unroll
     for (int p = 0; p < P; p++) x[p] = nx[p];
+}
+
+/* copies the inverse normal's segment table into LDS (all threads of the
+ * workgroup; the caller's next barrier publishes it) */
+__device__ __forceinline__ void load_ncdf_table(NcdfSeg *sh) {
+    const NcdfSeg *tab = ncdf_table();
+    for (int k = threadIdx.x; k < GF_NCDF_SEGMENTS; k += WG) sh[k] = tab[k];
 }
 
 /* locate a slot born inside the block: generation index t and population */
@@ -221,6 +233,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int w = threadIdx.x >> 6;
     __shared__ int sh_tile;
     __shared__ unsigned int sh_wc[WG / 64];
+    __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
+    load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
     uint64_t cnt = 0;        /* one population: lane t counts generation t */
     uint64_t wave_cnt = 0;   /* batches: the wave's live mutation-generations */
     /* tiles are handed out dynamically (an atomic ticket): their cost varies
@@ -279,7 +293,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             const int g = S.g0 + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
-                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel());
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf);
             }
             keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             const uint64_t b = __ballot(keep);
@@ -402,6 +416,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     __shared__ int sh_pre[WG + 1];
     __shared__ unsigned long long sh_live[64];
     __shared__ int sh_diff[KMAX + 1];
+    __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
+    load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
     if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
     if (threadIdx.x <= KMAX) sh_diff[threadIdx.x] = 0;
     const int M0 = *F.M_in;
@@ -474,7 +490,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             const int g = S.g0 + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
-                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel);
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel, sh_ncdf);
             }
             keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             if constexpr (P == 1) {

@@ -160,51 +160,37 @@ GF_HD float sqrt_pos(float x) {
 }
 
 /* Inverse standard normal CDF for p in [2^-33, 1] (the range of u01; p is
- * clamped to <= 1 - 2^-24 so the result stays finite).  Method: the change
- * of variable of Giles ("Approximating the erfinv function", GPU Computing
- * Gems Jade Edition, 2011): with s = min(p, 1 - p) and w = -log2(4 s (1 - s)),
- * ndtri(p) / (p - 1/2) is a smooth function of w, given by one polynomial
- * for w < 8 (p within about 9.8e-4 of 0 or 1 is the tail) and one in
- * sqrt(w) beyond.  No division; the tail branch runs in few waves.
- * Coefficients fitted by parallelpopgen_amd/csrc/fit_normcdfinv.py; error
- * against scipy.special.ndtri below 3.4e-7 relative (|z| > 1e-3) and
- * 1.6e-6 absolute over every u01 value tested. */
-static constexpr float kNcdfLog2[10] = {
-    0x1.715476p+0f, -0x1.71547p-1f, 0x1.ec70acp-2f, -0x1.715a6ep-2f, 0x1.2779eep-2f,
-    -0x1.eab71ep-3f, 0x1.a399dep-3f, -0x1.880824p-3f, 0x1.7a19cap-3f, -0x1.b74d82p-4f};
-static constexpr float kNcdfCentral[10] = {
-    0x1.1be4ccp+2f, 0x1.ea0ccp-2f, -0x1.c6ba7p-8f, -0x1.f793d2p-11f, 0x1.1d1adcp-13f,
-    -0x1.2aad7ap-18f, -0x1.b3f612p-21f, 0x1.bf18f8p-24f, 0x1.ff977ep-31f, -0x1.a7008p-31f};
-static constexpr float kNcdfTail[9] = {
-    0x1.2dd49ap+3f, 0x1.2fa2d4p+1f, 0x1.a6a788p-8f, -0x1.b5be06p-9f, 0x1.76b48ep-10f,
-    -0x1.b1259cp-11f, 0x1.fbb8cap-12f, -0x1.3a9bp-13f, 0x1.6a392ep-17f};
+ * clamped to <= 1 - 2^-24 so the result stays finite), from a segment
+ * table (gf_ncdf_table.h, fitted by parallelpopgen_amd/csrc/fit_normcdfinv.py):
+ * with s = min(p, 1 - p) in [2^-33, 1/2], the float bits of s pick one of
+ * 16 segments per binade (biased exponent and top 4 mantissa bits; the
+ * segments are logarithmically spaced towards the tail) and the low 19
+ * mantissa bits, read as an integer t, feed that segment's cubic
+ * c0 + t (c1 + t (c2 + t c3)) approximating ndtri(s); the result is r for
+ * p < 1/2 and -r above.  No division, logarithm or branch.  Error against
+ * scipy.special.ndtri over u01's values: at most 1.5 ulp of the result
+ * where |z| >= 0.1, 4.8e-7 absolute everywhere.  `tab` is the table itself
+ * on the host and in the library's less frequent kernels, and a copy in LDS
+ * in the drift kernels (8 KB). */
+#include "gf_ncdf_table.h"
 
-/* c[0] + x (c[1] + x (c[2] + ...)), fused multiply-adds */
-template <int N>
-GF_HD float horner_fma(const float (&c)[N], float x) {
-    float acc = c[N - 1];
-#pragma unroll
-    for (int k = N - 2; k >= 0; k--) acc = __builtin_fmaf(acc, x, c[k]);
-    return acc;
-}
+struct alignas(16) NcdfSeg { float c0, c1, c2, c3; };
 
-GF_HD float normcdfinv_f(float p) {
+GF_HD const NcdfSeg *ncdf_table() { return reinterpret_cast<const NcdfSeg *>(kNcdfSeg); }
+
+GF_HD float normcdfinv_tab(const NcdfSeg *tab, float p) {
     GF_STRICT_FP
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
-    const float s = q < 0.0f ? p : 1.0f - p;        /* (0, 1/2], exact */
-    const float v = (4.0f * s) * (1.0f - s);        /* (0, 1], normal */
-    const uint32_t b = f2bits(v);
-    int e = (int)(b >> 23) - 127;
-    float m = bits2f((b & 0x007fffffu) | 0x3f800000u);
-    if (m > 1.41421356f) { m = 0.5f * m; e += 1; }
-    const float f = m - 1.0f;
-    const float w = -(float)e - f * horner_fma(kNcdfLog2, f);   /* -log2(v) >= 0 */
-    float r;
-    if (w < 8.0f) r = horner_fma(kNcdfCentral, w - 4.0f);
-    else r = horner_fma(kNcdfTail, sqrt_pos(w) - 0x1.0cff86p+2f);
-    return q * r;
+    const float s = q < 0.0f ? p : 1.0f - p;        /* [2^-33, 1/2], exact */
+    const uint32_t b = f2bits(s);
+    const NcdfSeg c = tab[(b >> GF_NCDF_T_BITS) - GF_NCDF_BASE];
+    const float t = (float)(b & ((1u << GF_NCDF_T_BITS) - 1u));
+    const float r = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.c3, t, c.c2), t, c.c1), t, c.c0);
+    return q < 0.0f ? r : -r;
 }
+
+GF_HD float normcdfinv_f(float p) { return normcdfinv_tab(ncdf_table(), p); }
 
 GF_HD float round_away(float v) {
     GF_STRICT_FP
@@ -305,11 +291,12 @@ GF_HD float exp_f_mid(float x) {
  * comes first, or 70.  Callers pass 0 <= mean <= 6 (the Poisson tails of
  * the approximate binomial), where exp_f_mid(-mean) == exp_f(-mean).  The
  * loop stays rolled (lanes leave it as they finish) and takes four
- * reciprocals per trip in one scalar load; a trip computes its four terms
- * straight through and then picks the first that stops, so a wave changes
- * its lane mask once per trip instead of once per term (the terms past the
- * stopping one are computed and discarded; the recurrence and the
- * comparisons are those of one term at a time). */
+ * reciprocals per trip in one scalar load.  A trip computes its four terms
+ * straight through; the partial sums d0 <= d1 <= d2 <= d3 are
+ * non-decreasing (every term is >= 0), so the trip holds the answer iff
+ * d3 >= r or the early exit falls inside it, and then the answer is the
+ * number of partial sums below r, capped at the early exit.  The
+ * recurrence and the comparisons are those of one term at a time. */
 GF_HD int poiscdfinv(float r, float mean) {
     GF_STRICT_FP
     float emu = exp_f_mid(-1.0f * mean);
@@ -328,13 +315,12 @@ GF_HD int poiscdfinv(float r, float mean) {
         const float e1 = e0 * (mean * c1), d1 = d0 + e1;
         const float e2 = e1 * (mean * c2), d2 = d1 + e2;
         const float e3 = e2 * (mean * c3), d3 = d2 + e3;
-        const int rel = jstop - j;   /* the early exit, relative to this trip */
-        int k = (d3 >= r || rel == 3) ? 3 : 4;
-        k = (d2 >= r || rel == 2) ? 2 : k;
-        k = (d1 >= r || rel == 1) ? 1 : k;
-        if (j == 69) k = (d0 >= r || rel == 0) ? 0 : 1;   /* term 69 is the last: 70 after it */
-        else k = (d0 >= r || rel == 0) ? 0 : k;
-        if (k < 4) return j + k;
+        const int rel = jstop - j;   /* the early exit, relative to this trip (>= 0; <= 1 at j = 69,
+                                      * whose three padding terms are 0) */
+        if (d3 >= r || rel <= 3) {
+            const int k = (int)(d0 < r) + (int)(d1 < r) + (int)(d2 < r) + (int)(d3 < r);
+            return j + (k < rel ? k : rel);
+        }
         emu = e3;
         cdf = d3;
     }
@@ -359,20 +345,33 @@ static __host__ __device__ __attribute__((noinline)) int binomcdfinv(float r, fl
     return 70;
 }
 
+/* round_away(v) as an integer, for the caller's clamp to [0, N]: equal to
+ * f2i_sat(round_away(v)) for v >= 0; <= 0 for v < 0, like round_away */
+GF_HD int round_away_nonneg(float v) {
+    GF_STRICT_FP
+    const float d = v - __builtin_truncf(v);
+    return f2i_sat(v) + (d >= 0.5f ? 1 : 0);
+}
+
 /* approximate binomial draw (reference shared.cu:10-18): the two Poisson
  * (or binomial, N < 100) tails share one CDF inversion; the rest is the
- * rounded normal approximation */
-GF_HD int binom_draw(uint32_t bits, float mean, float var, float N) {
+ * rounded normal approximation.  A mean of exactly 0 or N (lost or fixed:
+ * the Poisson inversion of mean 0 returns 0) is answered without entering
+ * the inversion.  The result equals the reference's draw wherever that is
+ * >= 0; where the normal approximation goes below 0 it is some value <= 0
+ * (round_away_nonneg), and the caller clamps the result to [0, N]. */
+GF_HD int binom_draw(uint32_t bits, float mean, float var, float N, const NcdfSeg *ncdf = ncdf_table()) {
     GF_STRICT_FP
     const float u = u01(bits);
     const bool lo = mean <= 6.0f;
     const bool hi = !lo && (mean >= N - 6.0f);
     if (lo || hi) {
         const float m = lo ? mean : N - mean;
-        const int k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv(u, m);
+        int k = 0;
+        if (m != 0.0f) k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv(u, m);
         return lo ? k : f2i_sat(N - (float)k);
     }
-    return f2i_sat(round_away(normcdfinv_f(u) * sqrt_pos(var) + mean));
+    return round_away_nonneg(normcdfinv_tab(ncdf, u) * sqrt_pos(var) + mean);
 }
 
 /* approximate Poisson draw (reference shared.cuh:251-256) */

@@ -2,8 +2,8 @@
  * gf_spectrum.hpp -- site-frequency-spectrum kernels.
  *
  * k_pop_hist: integer histogram of round(Nc * x) with bin Nc folded into 0
- *   (reference population_hist, spectrum.cu:74-83); LDS-privatised when the
- *   histogram fits (Nc <= 16384), global 32-bit atomics otherwise.
+ *   (reference population_hist, spectrum.cu:74-83), LDS-privatised in passes
+ *   over bin ranges of up to 40,960 bins.
  * k_sfs: expected SFS of an n-chromosome sample, sum over segregating
  *   mutations of exp(log C(n,k) + k log p + (n-k) log q) for k = 0..n
  *   (reference binom, spectrum.cu:111-139).  Each term is truncated to 2^-32
@@ -21,30 +21,34 @@
 
 namespace gf {
 
-constexpr int HIST_LDS_BINS = 16384;
 constexpr int SFS_LDS_BINS = 4096;
 
-template <bool LDS>
-__global__ __launch_bounds__(WG) void k_pop_hist(const float *__restrict__ freq, int M, int Nc,
-                                                 unsigned int *__restrict__ hist) {
-    __shared__ unsigned int sh[LDS ? HIST_LDS_BINS : 1];
-    if (LDS) {
-        for (int k = threadIdx.x; k < Nc; k += WG) sh[k] = 0;
-        __syncthreads();
-    }
-    for (int i = blockIdx.x * WG + threadIdx.x; i < M; i += gridDim.x * WG) {
+/* Population histogram in LDS passes over bin ranges: workgroup (x, y)
+ * counts the mutations of strip x (a grid-stride share of the row) whose bin
+ * lies in range y, [y * chunk, y * chunk + chunk), in its own LDS copy of
+ * that range (dynamic LDS, chunk <= 40,960 bins = 160 KB), then adds its
+ * non-zero bins to the global histogram.  Every row is read once per bin
+ * range (Nc = 4e5 takes ten), and global atomics touch only non-zero
+ * per-strip bins. */
+constexpr int HIST_WG = 1024;
+constexpr int HIST_MAX_CHUNK = 40960;
+
+static __global__ __launch_bounds__(HIST_WG) void k_pop_hist(const float *__restrict__ freq, int M, int Nc, int chunk,
+                                                      unsigned int *__restrict__ hist) {
+    extern __shared__ unsigned int sh_hist[];
+    const int b0 = (int)blockIdx.y * chunk;
+    const int nb = Nc - b0 < chunk ? Nc - b0 : chunk;
+    for (int k = threadIdx.x; k < nb; k += HIST_WG) sh_hist[k] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * HIST_WG + threadIdx.x; i < M; i += gridDim.x * HIST_WG) {
         int idx = f2i_sat(round_away((float)Nc * freq[i]));
         if (idx == Nc) idx = 0;
-        if (idx >= 0 && idx < Nc) {
-            if (LDS) atomicAdd(&sh[idx], 1u);
-            else atomicAdd(&hist[idx], 1u);
-        }
+        const unsigned int r = (unsigned int)(idx - b0);
+        if (idx >= 0 && r < (unsigned int)nb) atomicAdd(&sh_hist[r], 1u);
     }
-    if (LDS) {
-        __syncthreads();
-        for (int k = threadIdx.x; k < Nc; k += WG)
-            if (sh[k]) atomicAdd(&hist[k], sh[k]);
-    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += HIST_WG)
+        if (sh_hist[k]) atomicAdd(&hist[b0 + k], sh_hist[k]);
 }
 
 /* acc layout per segment: [0..n] fixed-point level sums, [n+1] count of

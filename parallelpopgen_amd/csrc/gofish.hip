@@ -140,6 +140,28 @@ static int check_params(const gf_sim_desc *d, int t, int g, const std::vector<ui
     return GF_OK;
 }
 
+/* The checks of check_params for every generation of a run, with the
+ * extinction bookkeeping of PlainRun::plan (a population goes extinct when
+ * its size drops to 0 after being positive), on the host only. */
+static int validate_desc(const gf_sim_desc *d) {
+    const int P = d->num_populations;
+    const int G = d->num_generations;
+    const int start = d->init_mode == GF_INIT_PREV ? d->start_generation : 0;
+    std::vector<uint8_t> ext(P, 0);
+    if (d->init_mode == GF_INIT_PREV && d->init_num_mutations > 0 && d->init_extinct)
+        for (int p = 0; p < P; p++) ext[p] = d->init_extinct[p] ? 1 : 0;
+    std::vector<int> prevN(P, 0);
+    for (int t = 0; t <= G; t++) {
+        GF_TRY(check_params(d, t, start + t, ext));
+        const gf_pop_gen *pg = d->params + (size_t)t * P;
+        for (int p = 0; p < P; p++) {
+            if (t > 0 && !ext[p] && pg[p].N <= 0 && prevN[p] > 0) ext[p] = 1;
+            prevN[p] = pg[p].N;
+        }
+    }
+    return GF_OK;
+}
+
 /* ---------------- one simulation ---------------- */
 template <int P>
 struct PlainRun {
@@ -774,9 +796,8 @@ int32_t gf_device_count(int32_t *out) {
     return GF_OK;
 }
 
-static int32_t run_any(const gf_sim_desc *d, const gf_selection_hooks *hooks, gf_result **out) {
-    if (!out) { gf::set_error("gf_run: out is NULL"); return GF_ERR_ARG; }
-    *out = nullptr;
+/* the descriptor checks shared by gf_run and gf_check_sim */
+static int32_t check_desc_args(const gf_sim_desc *d) {
     if (!d || !d->params || !d->migration || !d->preserve || !d->sample) {
         gf::set_error("gf_run: descriptor or one of its tables is NULL");
         return GF_ERR_ARG;
@@ -790,6 +811,18 @@ static int32_t run_any(const gf_sim_desc *d, const gf_selection_hooks *hooks, gf
         gf::set_error("gf_run: invalid num_generations, init_mode, compact_interval or init_num_mutations");
         return GF_ERR_ARG;
     }
+    return GF_OK;
+}
+
+int32_t gf_check_sim(const gf_sim_desc *d) {
+    GF_TRY(check_desc_args(d));
+    return gf::validate_desc(d);
+}
+
+static int32_t run_any(const gf_sim_desc *d, const gf_selection_hooks *hooks, gf_result **out) {
+    if (!out) { gf::set_error("gf_run: out is NULL"); return GF_ERR_ARG; }
+    *out = nullptr;
+    GF_TRY(check_desc_args(d));
     std::unique_ptr<gf_result> R(new gf_result);
     int rc;
     switch (d->num_populations) {

@@ -115,6 +115,31 @@ int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *
     return GF_OK;
 }
 
+/* k_pop_hist over every bin range of [0, Nc): bin ranges as large as the
+ * workgroup's LDS allows (at most HIST_MAX_CHUNK bins), and as many strips
+ * per range as fill the device about twice over, each strip at least 8K
+ * mutations */
+static hipError_t launch_pop_hist(const float *d_freq, int M, int Nc, unsigned int *d_hist, hipStream_t st) {
+    static int lds_bytes = -1;
+    if (lds_bytes < 0) {
+        int dev = 0, v = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e != hipSuccess) return e;
+        v = std::min(v, HIST_MAX_CHUNK * (int)sizeof(unsigned int));
+        e = hipFuncSetAttribute((const void *)k_pop_hist, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+        if (e != hipSuccess) return e;
+        lds_bytes = v;
+    }
+    const int chunk = std::min(Nc, lds_bytes / (int)sizeof(unsigned int));
+    const int ranges = (Nc + chunk - 1) / chunk;
+    const int want = std::max(1, 2 * num_cus() / ranges);
+    const int strips = std::max(1, std::min(want, (M + 8191) / 8192));
+    hipLaunchKernelGGL(k_pop_hist, dim3(strips, ranges), dim3(HIST_WG), (size_t)chunk * sizeof(unsigned int), st,
+                       d_freq, M, Nc, chunk, d_hist);
+    return hipGetLastError();
+}
+
 } /* namespace gf */
 
 using namespace gf;
@@ -141,12 +166,7 @@ int32_t gf_population_histogram(const float *freq, int32_t M, int32_t Nc, float 
             if (M > 0) e = hipMemcpyAsync(d_f.p, freq, (size_t)M * sizeof(float), hipMemcpyHostToDevice, st);
             if (e == hipSuccess) e = hipMemsetAsync(d_h.p, 0, (size_t)Nc * sizeof(unsigned int), st);
             if (e == hipSuccess && M > 0) {
-                const int grid = std::min((M + WG - 1) / WG, num_cus() * 4);
-                if (Nc <= HIST_LDS_BINS)
-                    hipLaunchKernelGGL((k_pop_hist<true>), dim3(grid), dim3(WG), 0, st, d_f.p, M, Nc, d_h.p);
-                else
-                    hipLaunchKernelGGL((k_pop_hist<false>), dim3(grid), dim3(WG), 0, st, d_f.p, M, Nc, d_h.p);
-                e = hipGetLastError();
+                e = launch_pop_hist(d_f.p, M, Nc, d_h.p, st);
             }
             if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_h.p, (size_t)Nc * sizeof(unsigned int), hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);

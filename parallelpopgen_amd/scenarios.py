@@ -22,6 +22,34 @@ def _f32(x):
     return np.float32(x)
 
 
+_LIBM = None
+
+
+def _libm():
+    """the C library's float functions, which the C++ API's functors call
+    on the host (expf in demography_exponential_growth, sinf in the sine-wave
+    models): numpy's float32 exp and sin may differ from them in the last
+    place"""
+    global _LIBM
+    if _LIBM is None:
+        import ctypes
+        import ctypes.util
+        lib = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        for name in ("expf", "sinf"):
+            getattr(lib, name).restype = ctypes.c_float
+            getattr(lib, name).argtypes = [ctypes.c_float]
+        _LIBM = lib
+    return _LIBM
+
+
+def expf(x):
+    return np.float32(_libm().expf(float(np.float32(x))))
+
+
+def sinf(x):
+    return np.float32(_libm().sinf(float(np.float32(x))))
+
+
 def selection_from_gamma(gamma, N0, F0):
     """s = gamma / (2*N/(1+F)) in float, as Sim_Model::selection_constant's
     effective-selection constructor (template_inline_simulation_functors.cuh:52)."""
@@ -89,7 +117,8 @@ def three_population(seed1=0xbeeff00d, seed2=SEED2_DEFAULT, G=2000, L=2e8, N0=20
     params = np.zeros((G + 1, P), POP_GEN_DTYPE)
     rate = _f32(math.log(10.0) / 500.0)
     g = np.arange(G + 1)
-    grow = np.array([int(np.round(_f32(N0) * np.exp(_f32(rate * _f32(t))))) for t in g], np.int64)
+    # (int)roundf(N0 * expf(rate * t)), half away from zero
+    grow = np.array([int(math.floor(float(_f32(_f32(N0) * expf(_f32(rate * _f32(t))))) + 0.5)) for t in g], np.int64)
     grow[g >= 500] = N0 * 10
     params["N"][:, 0] = grow
     params["N"][:, 1] = np.where((g >= 500) & (g < 600), N0 // 10, N0)
@@ -99,7 +128,7 @@ def three_population(seed1=0xbeeff00d, seed2=SEED2_DEFAULT, G=2000, L=2e8, N0=20
     params["mu"] = 1e-9
     A = _f32(_f32(5.0) / _f32(_f32(2 * N0) / _f32(1.0)))
     pi = _f32(2 * math.pi / 200.0)
-    s_t = (A * np.sin((pi * g.astype(np.float32)).astype(np.float32)).astype(np.float32)).astype(np.float32)
+    s_t = np.array([_f32(_f32(A * sinf(_f32(pi * _f32(t)))) + _f32(0.0)) for t in g], np.float32)
     params["sel_slope"] = 0.0
     params["sel_intercept"] = s_t[:, None]
     mig = np.broadcast_to(migration_constant_equal(1e-4, P), (G + 1, P, P)).copy()

@@ -25,6 +25,12 @@ def run_example(name, *args, timeout=600):
     exe = os.path.join(BUILD, name)
     if not os.path.exists(exe):
         pytest.fail("example %s is not built (make -C examples)" % name)
+    # the binary must be current: make's question mode exits non-zero if a
+    # source or header is newer than what was built from it
+    q = subprocess.run(["make", "-q", "-C", os.path.join(REPO, "examples"), os.path.join(BUILD, name)],
+                       capture_output=True, text=True)
+    if q.returncode != 0:
+        pytest.fail("example %s is older than its sources (make -C examples)" % name)
     r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout, cwd=BUILD)
     assert r.returncode == 0, r.stderr[-2000:]
     return r.stdout

@@ -85,7 +85,7 @@ def test_positive_selection_dominance_inbreeding(gpu, oracle):
     assert_same(gpu.run(d), oracle.run(d), "haploid")
 
 
-def split_scenario(G=400, L=5e7, split=100):
+def split_scenario(G=400, L=5e7, split=100, interval=25):
     """population 1 splits off population 0 at generation `split`, grows,
     then both exchange migrants; population 0 goes extinct at generation
     300.  A split generation that is not a multiple of 4 founds the
@@ -111,7 +111,7 @@ def split_scenario(G=400, L=5e7, split=100):
         mig[t] = [[1 - m, m], [m, 1 - m]]
     sample = np.zeros(G + 1, np.uint8)
     sample[[150, 299]] = 1
-    return SimDesc(P, L, 0x1234, 0x5678, G, params, mig, None, sample, 25)
+    return SimDesc(P, L, 0x1234, 0x5678, G, params, mig, None, sample, interval)
 
 
 @pytest.mark.parametrize("split", [100, 101, 103])
@@ -120,6 +120,29 @@ def test_split_migration_extinction(gpu, oracle, split):
     g, o = gpu.run(d), oracle.run(d)
     assert_same(g, o, "split at %d" % split)
     assert g["samples"][-1]["extinct"] == [True, False]
+
+
+_split_ref = {}
+
+
+@pytest.mark.parametrize("interval", [0, 25, 100])
+@pytest.mark.parametrize("kpl", [0, 1, 20, 64])
+def test_several_populations_launch_length(gpu, oracle, interval, kpl):
+    """two populations (split at 103: inside a block of four generations
+    sharing one Philox output) with launches of 1, 20 and 64 generations and
+    the library's own choice (0), compaction every 25 or 100 generations or
+    never: every launch length gives the oracle's run bit for bit, so
+    per-generation live counters in lanes up to 63, a population founded
+    inside a long launch and tile tickets over long launches are covered"""
+    d = split_scenario(split=103, interval=interval)
+    if interval not in _split_ref:
+        _split_ref[interval] = oracle.run(d)
+    gpu.set_generations_per_launch(kpl)
+    try:
+        g = gpu.run(d)
+    finally:
+        gpu.set_generations_per_launch(0)
+    assert_same(g, _split_ref[interval], "interval=%d kpl=%d" % (interval, kpl))
 
 
 def test_prev_sim_restart(gpu, oracle):
@@ -229,3 +252,23 @@ def test_large_launch_chunked_tile_scan(gpu, oracle):
         oracle.set_threads(1)
     assert g["samples"][0]["num_mutations"] > 6_000_000
     assert_same(g, o, "chunked scan")
+
+
+@pytest.mark.parametrize("Nc", [10000, 16384, 20000, 40961, 200000, 400000])
+def test_population_histogram_bin_ranges(gpu, oracle, Nc):
+    """population histogram in LDS passes over bin ranges: one range
+    (Nc = 1e4), the old single-pass limit (16,384), a range boundary
+    (40,961 = one full range + one bin) and the three-population scenario's
+    sizes up to 4e5 bins; frequencies on the 1/Nc grid, off it, 0, 1 and
+    the half-way points that round away from zero; GPU = oracle"""
+    rng = np.random.default_rng(Nc)
+    M = 3_000_000
+    grid = rng.integers(0, Nc + 1, M // 2) / Nc
+    neutral = np.exp(rng.uniform(np.log(1.0 / Nc), 0.0, M - M // 2 - 4))
+    half = (np.array([0.5, 1.5, Nc - 0.5, Nc / 2 + 0.5]) / Nc)
+    row = np.concatenate([grid, neutral, half]).astype(np.float32)
+    row[:7] = [0.0, 1.0, 0.0, 1.0, 1.0 / Nc, (Nc - 1.0) / Nc, 0.5]
+    h_g = gpu.population_frequency_histogram(row, Nc, 5e8)
+    h_o = oracle.population_frequency_histogram(row, Nc, 5e8)
+    assert h_g.shape == (Nc,)
+    assert np.array_equal(h_g, h_o)
