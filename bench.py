@@ -19,7 +19,14 @@ Prints ONE JSON line (rank 0, N = 1).  Also reports:
                 FETCH_SIZE/WRITE_SIZE (profiles/advance_traffic.json), if any
   cpu_baseline  the CPU oracle (oracle/, a restatement of the same
                 algorithm) on the same scenario for a bounded prefix of
-                generations, on the host's cores
+                generations, on the host's cores (CPU model and thread
+                counts stated)
+  batched       the batched parameter sweep below (4,096 simulations)
+                against the same simulations back to back (sims/s)
+  surrogate     the surrogate below, trained on 262,144 draws: predictions/s
+                against the batched simulator's SFS/s, and the held-out
+                deviance ratio against two simulation replicates
+                (--plain-only leaves both out)
 
 Other workloads (--workload), each printing one JSON line of its own:
   three_population
@@ -78,6 +85,12 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=32, help="batched: simulations timed back to back")
     ap.add_argument("--train-draws", type=int, default=1000000, help="surrogate: training draws")
     ap.add_argument("--test-draws", type=int, default=10000, help="surrogate: held-out draws")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="default workload: report the plain run only (no batched / surrogate sub-objects)")
+    ap.add_argument("--default-train-draws", type=int, default=262144,
+                    help="default workload's surrogate part: training draws")
+    ap.add_argument("--default-test-draws", type=int, default=2000,
+                    help="default workload's surrogate part: held-out draws")
     return ap.parse_args()
 
 
@@ -92,10 +105,31 @@ def gpu_step(capi, scenarios, G):
     return dt, mutgens, r
 
 
+def cpu_model():
+    """the host CPU's model name and hardware thread count"""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return name, os.cpu_count() or 1, usable
+
+
 def cpu_baseline(scenarios, G_gpu, want_gens):
-    """Times the oracle on the same scenario, first `want_gens` generations."""
+    """Times the oracle on the same scenario, first `want_gens` generations,
+    on at most 16 of the host's threads (the share of the host a one-GPU box
+    gives this process)."""
     from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, hw_threads, usable = cpu_model()
+    threads = max(1, min(16, usable))
     O.set_threads(threads)
     G = want_gens
     if G <= 0:
@@ -111,6 +145,7 @@ def cpu_baseline(scenarios, G_gpu, want_gens):
     dt = time.perf_counter() - t0
     mutgens = int(r["live"][1:].sum())
     return dict(value=mutgens / dt, unit="mutation-generations/s", cores=threads, kind="port",
+                cpu_model=model, hardware_threads=hw_threads, usable_threads=usable,
                 sample="plain_large scenario (N=1e5, L=5e8, gamma=-10), equilibrium start + first %d of %d "
                        "generations, %d threads, %.1f s" % (G, G_gpu, threads, dt))
 
@@ -428,6 +463,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": "FETCH_SIZE + WRITE_SIZE per launch of this kernel on this workload, from "
+                              "rocprofv3 --pmc runs (profiles/advance_traffic.json), not measured in this run",
             "kernel": "k_advance_fused (migration_selection_drift + add_new_mutations + tile-local compaction, "
                       "%.1f generations per launch)" % (a.generations * a.steps / max(adv_launch, 1)),
             "avg_launch_ms": avg_launch_ms,
@@ -437,6 +474,25 @@ def main():
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scenarios, a.generations, a.cpu_generations)
+    if not a.plain_only:
+        # the other two parts of the metric, on the same GPU in the same run
+        b = bench_batched(a, capi, scenarios)
+        out["batched"] = {
+            "metric": "simulations/s", "value": b["value"], "unit": "simulations/s",
+            "seconds": b["ms_per_step"] / 1e3, "workload": b["config"]["workload"],
+            "mutation_generations_per_s": b["config"]["mutation_generations_per_s"],
+            "plain_back_to_back": b["plain_back_to_back"],
+        }
+        a.train_draws, a.test_draws = a.default_train_draws, a.default_test_draws
+        sg = bench_surrogate(a)
+        out["surrogate"] = {
+            "metric": "SFS/s", "value": sg["value"], "unit": "SFS predictions/s",
+            "workload": sg["config"]["workload"],
+            "simulator_sfs_per_s": sg["simulator_sfs_per_s"],
+            "speedup_vs_batched_simulator": sg["speedup_vs_batched_simulator"],
+            "deviance_ratio": sg["deviance_ratio"], "held_out": sg["held_out"],
+            "training_seconds": sg["training_seconds"],
+        }
     print(json.dumps(out))
 
 

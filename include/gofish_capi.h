@@ -298,6 +298,13 @@ int32_t gf_sg_forward(gf_sg_model *model, const float *x, int32_t n, float *logi
 int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void *in2, const void *in3,
                        void *out, int64_t n, int32_t device);
 
+/* Frees the device memory the library keeps for reuse on `device` (-1:
+ * the current device).  Released device buffers are kept in a per-device
+ * cache of at most 16 GiB instead of being returned with hipFree, which
+ * waits for the whole device (no counterpart in the reference, which
+ * allocates and frees on every compaction, go_fish_impl.cuh:377-428). */
+int32_t gf_release_device_cache(int32_t device);
+
 /* ----- runtime ----- */
 const char *gf_last_error(void);
 /* 0: off; 1: every kernel launch is timed; s > 1: every s-th launch (a
@@ -309,6 +316,11 @@ int32_t gf_set_kernel_timing(int32_t enable);
  * 1.2e6, 10 beyond; several: 64 up to 2e6 slots, 20 beyond, a launch
  * ending at each compaction generation either way; batches: 16). */
 int32_t gf_set_generations_per_launch(int32_t k);
+/* Draw-path regrouping of the drift kernel: at the start of a launch, the
+ * slots of a tile whose count is within `threshold` of 0 or N (Poisson tails
+ * of the approximate binomial) are gathered into the same waves, and the
+ * rest into others; results are identical with any threshold (0: off). */
+int32_t gf_set_draw_regroup(float threshold);
 int32_t gf_device_count(int32_t *out);
 
 #ifdef __cplusplus

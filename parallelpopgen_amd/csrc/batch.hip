@@ -88,7 +88,7 @@ struct BatchRun {
     int S, interval, n;
     bool keep;
     gf_batch_result *R;
-    hipStream_t st = nullptr;
+    OwnedStream st;   /* before the buffers: they are released while it is alive */
     std::vector<BatchSimDev> sims;
     DevBuf<BatchSimDev> d_sims;
     DevBuf<float> freqA, freqB;
@@ -99,8 +99,6 @@ struct BatchRun {
     std::vector<int> M;
     std::vector<int> G;
     KernelTimer t_adv, t_cmp;
-
-    ~BatchRun() { if (st) (void)hipStreamDestroy(st); }
 
     int ensure_total(int64_t need) {
         if (need <= cap && freqA.p) return GF_OK;
@@ -248,7 +246,8 @@ struct BatchRun {
         if (keep) { R->freq.resize(S); R->ids.resize(S); }
         t_adv.stride = t_cmp.stride = timing_stride();
         auto t0 = std::chrono::steady_clock::now();
-        GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        GF_HIP(st.create());
+        StreamScope scope(st);
         hipEvent_t i0, i1;
         GF_HIP(hipEventCreate(&i0));
         GF_HIP(hipEventCreate(&i1));

@@ -15,6 +15,7 @@
 #include <vector>
 #include <chrono>
 #include <algorithm>
+#include <memory>
 #include "../../include/gofish_capi.h"
 
 namespace gf {
@@ -45,15 +46,66 @@ struct Status {
         if (rc_ != GF_OK) return rc_;    \
     } while (0)
 
+/* A library stream and its liveness.  Every entry point that issues device
+ * work creates an OwnedStream and makes it current for its thread
+ * (StreamScope); device buffers allocated meanwhile remember it, and on
+ * release record an event on it instead of waiting for the device.  An
+ * OwnedStream synchronises before it is destroyed and then marks its tag
+ * dead, so a buffer that outlives its stream has no pending work on it
+ * (every entry point also synchronises its stream before it returns). */
+struct StreamTag {
+    hipStream_t st = nullptr;
+    bool alive = true;
+};
+
+inline std::shared_ptr<StreamTag> &current_stream_tag() {
+    static thread_local std::shared_ptr<StreamTag> t;
+    return t;
+}
+
+struct OwnedStream {
+    hipStream_t st = nullptr;
+    std::shared_ptr<StreamTag> tag;
+    OwnedStream() = default;
+    OwnedStream(const OwnedStream &) = delete;
+    OwnedStream &operator=(const OwnedStream &) = delete;
+    hipError_t create() {
+        if (st) return hipSuccess;
+        const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) { st = nullptr; return e; }
+        tag = std::make_shared<StreamTag>();
+        tag->st = st;
+        return hipSuccess;
+    }
+    ~OwnedStream() {
+        if (!st) return;
+        (void)hipStreamSynchronize(st);
+        tag->alive = false;
+        (void)hipStreamDestroy(st);
+    }
+    operator hipStream_t() const { return st; }
+};
+
+/* makes `s` the calling thread's current library stream for its lifetime */
+struct StreamScope {
+    std::shared_ptr<StreamTag> prev;
+    explicit StreamScope(const OwnedStream &s) : prev(current_stream_tag()) { current_stream_tag() = s.tag; }
+    ~StreamScope() { current_stream_tag() = prev; }
+};
+
 /* Per-device cache of released device blocks.  hipFree waits for the whole
  * device and took up to 80 ms for large blocks (a batched run releases
- * about 60 buffers); a released block is kept instead, after the same
- * device-wide wait, so no block is handed out while earlier work may still
- * use it.  An allocation takes the smallest cached block of its size class
- * up to twice its size.  At most kKeep bytes are kept per device (the
- * oldest blocks are freed first); blocks above kMaxBlock are freed at
- * once; a failed hipMalloc empties the device's cache and tries again.
- * The cache lives until the process ends. */
+ * about 60 buffers); a released block is kept instead.  A block released
+ * by a buffer whose stream is alive carries an event recorded on that
+ * stream at release, and is handed out again only after that event
+ * completes (no device-wide wait); a block whose stream has ended is free
+ * at once; a block of unknown stream waits for the device at release.  An
+ * allocation takes the smallest cached block of its size class up to twice
+ * its size, preferring blocks whose event has completed.  At most kKeep
+ * bytes are kept per device (the oldest blocks are freed first); blocks
+ * above kMaxBlock are freed at once; a failed hipMalloc empties the
+ * device's cache and tries again; gf_release_device_cache empties it on
+ * request.  The cache lives until the process ends. */
 class DevCache {
 public:
     static constexpr size_t kKeep = (size_t)16 << 30;
@@ -76,21 +128,36 @@ public:
         hipError_t e = hipGetDevice(&dev);
         if (e != hipSuccess) return e;
         block = size_class(bytes);
+        hipEvent_t wait = nullptr;
         {
             std::lock_guard<std::mutex> lock(mu_);
             Dev &D = dev_(dev);
             size_t best = D.blocks.size();
+            bool best_ready = false;
             for (size_t i = 0; i < D.blocks.size(); i++) {
                 const size_t b = D.blocks[i].bytes;
-                if (b >= block && b <= 2 * block && (best == D.blocks.size() || b < D.blocks[best].bytes)) best = i;
+                if (b < block || b > 2 * block) continue;
+                const bool ready = !D.blocks[i].ev || hipEventQuery(D.blocks[i].ev) == hipSuccess;
+                if (best == D.blocks.size() || (ready && !best_ready) ||
+                    (ready == best_ready && b < D.blocks[best].bytes)) {
+                    best = i;
+                    best_ready = ready;
+                }
             }
             if (best < D.blocks.size()) {
                 *p = D.blocks[best].p;
                 block = D.blocks[best].bytes;
+                wait = D.blocks[best].ev;
                 D.kept -= block;
                 D.blocks.erase(D.blocks.begin() + (ptrdiff_t)best);
-                return hipSuccess;
             }
+        }
+        if (*p) {
+            if (wait) {
+                e = hipEventSynchronize(wait);
+                recycle(wait);
+            }
+            return e;
         }
         e = hipMalloc(p, block);
         if (e != hipSuccess) {
@@ -101,40 +168,41 @@ public:
         return e;
     }
 
-    void release(void *p, size_t block, int dev) {
+    void release(void *p, size_t block, int dev, const std::shared_ptr<StreamTag> &tag) {
         int cur = dev;
         (void)hipGetDevice(&cur);
         if (cur != dev) (void)hipSetDevice(dev);
-        (void)hipDeviceSynchronize();
+        hipEvent_t ev = nullptr;
+        if (tag && tag->alive) {
+            ev = event();
+            if (ev && hipEventRecord(ev, tag->st) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipEventDestroy(ev);
+                ev = nullptr;
+                (void)hipStreamSynchronize(tag->st);
+            }
+        } else if (!tag) {
+            (void)hipDeviceSynchronize();
+        }
         if (block > kMaxBlock) {
+            if (ev) { (void)hipEventSynchronize(ev); recycle(ev); }
             (void)hipFree(p);
         } else {
-            std::vector<void *> drop;
+            std::vector<Block> drop;
             {
                 std::lock_guard<std::mutex> lock(mu_);
                 Dev &D = dev_(dev);
-                D.blocks.push_back({p, block});
+                D.blocks.push_back({p, block, ev});
                 D.kept += block;
                 while (D.kept > kKeep && !D.blocks.empty()) {
-                    drop.push_back(D.blocks.front().p);
+                    drop.push_back(D.blocks.front());
                     D.kept -= D.blocks.front().bytes;
                     D.blocks.erase(D.blocks.begin());
                 }
             }
-            for (void *q : drop) (void)hipFree(q);
+            free_blocks(drop);
         }
         if (cur != dev) (void)hipSetDevice(cur);
-    }
-
-private:
-    struct Block { void *p; size_t bytes; };
-    struct Dev { std::vector<Block> blocks; size_t kept = 0; };
-    std::mutex mu_;
-    std::vector<Dev> devs_;
-
-    Dev &dev_(int d) {
-        if ((size_t)d >= devs_.size()) devs_.resize((size_t)d + 1);
-        return devs_[(size_t)d];
     }
 
     /* frees every cached block of device `dev` (the current device) */
@@ -146,30 +214,79 @@ private:
             drop.swap(D.blocks);
             D.kept = 0;
         }
-        (void)hipDeviceSynchronize();
-        for (const Block &b : drop) (void)hipFree(b.p);
+        free_blocks(drop);
+    }
+
+private:
+    struct Block { void *p; size_t bytes; hipEvent_t ev; };
+    struct Dev { std::vector<Block> blocks; size_t kept = 0; };
+    std::mutex mu_;
+    std::vector<Dev> devs_;
+    std::vector<hipEvent_t> spare_;   /* completed events for reuse */
+
+    Dev &dev_(int d) {
+        if ((size_t)d >= devs_.size()) devs_.resize((size_t)d + 1);
+        return devs_[(size_t)d];
+    }
+
+    hipEvent_t event() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            if (!spare_.empty()) {
+                hipEvent_t e = spare_.back();
+                spare_.pop_back();
+                return e;
+            }
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return e;
+    }
+
+    void recycle(hipEvent_t e) {
+        std::lock_guard<std::mutex> lock(mu_);
+        spare_.push_back(e);
+    }
+
+    void free_blocks(std::vector<Block> &drop) {
+        for (const Block &b : drop) {
+            if (b.ev) { (void)hipEventSynchronize(b.ev); recycle(b.ev); }
+            (void)hipFree(b.p);
+        }
+        drop.clear();
     }
 };
 
-/* owning device buffer (n elements; the block under it may be larger) */
+/* owning device buffer (n elements; the block under it may be larger);
+ * remembers the calling thread's current library stream at allocation */
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
     size_t block = 0;
     int dev = 0;
+    std::shared_ptr<StreamTag> tag;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) DevCache::get().release(p, block, dev);
+        if (p) DevCache::get().release(p, block, dev, tag);
         p = nullptr;
         n = 0;
         block = 0;
+        tag.reset();
     }
     int alloc(size_t count) {
-        if (count <= n && p) return GF_OK;
+        if (count <= n && p) {
+            /* reused under another call's stream: the earlier call
+             * synchronised its stream before it returned */
+            if (current_stream_tag()) tag = current_stream_tag();
+            return GF_OK;
+        }
         release();
         if (count == 0) count = 1;
         hipError_t e = DevCache::get().alloc((void **)&p, count * sizeof(T), block, dev);
@@ -179,6 +296,7 @@ struct DevBuf {
             set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
             return GF_ERR_ALLOC;
         }
+        tag = current_stream_tag();
         n = count;
         return GF_OK;
     }
@@ -187,6 +305,7 @@ struct DevBuf {
         std::swap(n, o.n);
         std::swap(block, o.block);
         std::swap(dev, o.dev);
+        std::swap(tag, o.tag);
     }
 };
 

@@ -46,11 +46,8 @@ struct gf_result {
     /* stream and scratch of the device-side SFS calls on this result (not
      * its data: mutable through the const accessors; like the reference's
      * API, one result is used by one host thread at a time) */
-    mutable hipStream_t sfs_stream = nullptr;
+    mutable gf::OwnedStream sfs_stream;
     mutable gf::SfsScratch sfs_scratch;
-    ~gf_result() {
-        if (sfs_stream) (void)hipStreamDestroy(sfs_stream);
-    }
 };
 
 namespace gf {
@@ -62,6 +59,7 @@ const std::string &get_error() { return t_err; }
 static int g_timing = 0;
 int timing_stride() { return g_timing; }
 static int g_kpl = 0;    /* generations per advance launch (1..KMAX); 0: the caller's default */
+static float g_regroup = 0.0f;   /* drift kernel's draw-path regrouping threshold; 0: off */
 int generations_per_launch(int dflt) { return g_kpl > 0 ? g_kpl : dflt; }
 
 int select_device(int device) {
@@ -168,7 +166,7 @@ struct PlainRun {
     const gf_sim_desc *d;
     const gf_selection_hooks *hooks = nullptr;   /* caller-compiled selection, or NULL: linear form */
     gf_result *R;
-    hipStream_t st = nullptr;
+    OwnedStream st;   /* before the buffers: they are released while it is alive */
     int G, start, final_g, interval;
     uint32_t k0, k1;
     std::vector<GenRow<P>> rows;          /* rows[t], t = 0..G */
@@ -208,8 +206,6 @@ struct PlainRun {
     int64_t launches = 0;
     int64_t M_bound = 0;                /* upper bound of the current slot count */
     int64_t last_new = 0;               /* new slots of the most recent launch */
-
-    ~PlainRun() { if (st) (void)hipStreamDestroy(st); }
 
     int plan() {
         const int Pn = d->num_populations;
@@ -554,6 +550,7 @@ struct PlainRun {
         F.dense = frag ? 0 : 1;
         F.compact = compact ? 1 : 0;
         F.preserve = preserve ? 1 : 0;
+        F.regroup = g_regroup;
         SegLaunch<P> sl{};
         sl.rows = d_rows.p + t_first;
         sl.live = d_live.p;
@@ -624,7 +621,8 @@ struct PlainRun {
         t_adv.stride = t_cmp.stride = timing_stride();
         auto t_start = std::chrono::steady_clock::now();
         GF_TRY(select_device(d->device));
-        GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        GF_HIP(st.create());
+        StreamScope scope(st);
         GF_TRY(plan());
         GF_TRY(d_rows.alloc(rows.size()));
         GF_HIP(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(GenRow<P>), hipMemcpyHostToDevice, st));
@@ -789,6 +787,23 @@ int32_t gf_set_generations_per_launch(int32_t k) {
     return GF_OK;
 }
 
+int32_t gf_set_draw_regroup(float threshold) {
+    if (!(threshold >= 0.0f)) { gf::set_error("draw regrouping threshold must be >= 0"); return GF_ERR_ARG; }
+    gf::g_regroup = threshold;
+    return GF_OK;
+}
+
+int32_t gf_release_device_cache(int32_t device) {
+    int cur = 0;
+    GF_HIP(hipGetDevice(&cur));
+    GF_TRY(gf::select_device(device));
+    int dev = 0;
+    GF_HIP(hipGetDevice(&dev));
+    gf::DevCache::get().flush(dev);
+    if (dev != cur) GF_HIP(hipSetDevice(cur));
+    return GF_OK;
+}
+
 int32_t gf_device_count(int32_t *out) {
     int n = 0;
     GF_HIP(hipGetDeviceCount(&n));
@@ -905,7 +920,8 @@ int32_t gf_result_site_frequency_spectrum(const gf_result *r, int32_t s, int32_t
     int cur = 0;
     GF_HIP(hipGetDevice(&cur));
     if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
-    if (!r->sfs_stream) GF_HIP(hipStreamCreateWithFlags(&r->sfs_stream, hipStreamNonBlocking));
+    GF_HIP(r->sfs_stream.create());
+    gf::StreamScope scope(r->sfs_stream);
     const int rc = gf::sfs_of_device_row(r->dev_freq.p + (size_t)pop * r->dev_pitch, x.num_mutations, n, num_sites, out,
                                          r->sfs_stream, &r->sfs_scratch);
     if (cur != r->dev_device) (void)hipSetDevice(cur);
@@ -927,11 +943,14 @@ int32_t gf_result_joint_site_frequency_spectrum(const gf_result *r, int32_t s, i
     int cur = 0;
     GF_HIP(hipGetDevice(&cur));
     if (cur != r->dev_device) GF_HIP(hipSetDevice(r->dev_device));
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    const int rc = gf::joint_sfs_of_device_rows(r->dev_freq.p + (size_t)pa * r->dev_pitch,
-                                                r->dev_freq.p + (size_t)pb * r->dev_pitch, M, na, nb, num_sites, out, st);
-    (void)hipStreamDestroy(st);
+    int rc;
+    {
+        gf::OwnedStream st;
+        GF_HIP(st.create());
+        gf::StreamScope scope(st);
+        rc = gf::joint_sfs_of_device_rows(r->dev_freq.p + (size_t)pa * r->dev_pitch,
+                                          r->dev_freq.p + (size_t)pb * r->dev_pitch, M, na, nb, num_sites, out, st);
+    }
     if (cur != r->dev_device) (void)hipSetDevice(cur);
     return rc;
 }

@@ -152,8 +152,9 @@ int32_t gf_population_histogram(const float *freq, int32_t M, int32_t Nc, float 
         return GF_ERR_ARG;
     }
     GF_TRY(select_device(device));
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    OwnedStream st;
+    GF_HIP(st.create());
+    StreamScope scope(st);
     int rc = GF_OK;
     {
         DevBuf<float> d_f;
@@ -181,7 +182,6 @@ int32_t gf_population_histogram(const float *freq, int32_t M, int32_t Nc, float 
             }
         }
     }
-    (void)hipStreamDestroy(st);
     return rc;
 }
 
@@ -191,8 +191,9 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t M, int32_t n, floa
         return GF_ERR_ARG;
     }
     GF_TRY(select_device(device));
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    OwnedStream st;
+    GF_HIP(st.create());
+    StreamScope scope(st);
     int rc = GF_OK;
     {
         DevBuf<float> d_f;
@@ -203,7 +204,6 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t M, int32_t n, floa
         }
         if (!rc) rc = sfs_of_device_row(d_f.p, M, n, num_sites, out, st);
     }
-    (void)hipStreamDestroy(st);
     return rc;
 }
 
@@ -214,8 +214,9 @@ int32_t gf_joint_site_frequency_spectrum(const float *freq_a, const float *freq_
         return GF_ERR_ARG;
     }
     GF_TRY(select_device(device));
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    OwnedStream st;
+    GF_HIP(st.create());
+    StreamScope scope(st);
     int rc = GF_OK;
     {
         DevBuf<float> d_f;
@@ -227,7 +228,6 @@ int32_t gf_joint_site_frequency_spectrum(const float *freq_a, const float *freq_
         }
         if (!rc) rc = joint_sfs_of_device_rows(d_f.p, d_f.p + M, M, n_a, n_b, num_sites, out, st);
     }
-    (void)hipStreamDestroy(st);
     return rc;
 }
 

@@ -289,7 +289,7 @@ using namespace gf::sg;
 struct gf_sg_model {
     int E = 4, Bmax = 0;
     Layout lay;
-    hipStream_t st = nullptr;
+    OwnedStream st;   /* before the buffers: they are released while it is alive */
     DevBuf<float> w, g, m, v;       /* [E][total] */
     DevBuf<bf16> wb;
     DevBuf<bf16> x;                 /* [B][64] */
@@ -302,7 +302,6 @@ struct gf_sg_model {
     DevBuf<float> psfs, ptot, pdis;
     int64_t step = 0;
     float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    ~gf_sg_model() { if (st) (void)hipStreamDestroy(st); }
 };
 
 namespace gf {
@@ -433,7 +432,8 @@ int32_t gf_sg_create(int32_t ensemble, uint64_t seed, int32_t device, gf_sg_mode
     GF_TRY(select_device(device));
     std::unique_ptr<gf_sg_model> S(new gf_sg_model);
     S->E = ensemble;
-    GF_HIP(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+    GF_HIP(S->st.create());
+    StreamScope scope(S->st);
     const size_t n = (size_t)S->E * S->lay.total;
     GF_TRY(S->w.alloc(n));
     GF_TRY(S->g.alloc(n));
@@ -498,6 +498,7 @@ int32_t gf_sg_set_params(gf_sg_model *S, const float *in) {
 int32_t gf_sg_gradients(gf_sg_model *S, const float *x, const float *y, const float *t, int32_t B, float w_tot,
                         float *grad_out, float *loss_out) {
     if (!S || !x || !y || !t || B <= 0 || B % TILE) { set_error("gf_sg_gradients: invalid arguments"); return GF_ERR_ARG; }
+    StreamScope scope(S->st);   /* buffers (re)allocated here belong to the model's stream */
     GF_TRY(ensure_batch(*S, B));
     GF_HIP(hipMemsetAsync(S->loss.p, 0, S->E * sizeof(float), S->st));
     GF_TRY(load_batch(*S, x, y, t, B));
@@ -514,6 +515,7 @@ int32_t gf_sg_gradients(gf_sg_model *S, const float *x, const float *y, const fl
 int32_t gf_sg_train(gf_sg_model *S, const float *x, const float *y, const float *t, int32_t n, int32_t epochs,
                     int32_t B, float lr, float w_tot, const int32_t *order, float *epoch_loss) {
     if (!S || !x || !y || !t || n < B || B <= 0 || B % TILE || epochs <= 0) { set_error("gf_sg_train: invalid arguments"); return GF_ERR_ARG; }
+    StreamScope scope(S->st);   /* buffers (re)allocated here belong to the model's stream */
     GF_TRY(ensure_batch(*S, B));
     /* stage the whole training set on the device once */
     DevBuf<float> dx, dy, dt;
@@ -561,6 +563,7 @@ int32_t gf_sg_train(gf_sg_model *S, const float *x, const float *y, const float 
  * mean log-total and the ensemble disagreement per input */
 int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, float *tot, float *dis) {
     if (!S || !x || n <= 0 || !sfs) { set_error("gf_sg_predict: invalid arguments"); return GF_ERR_ARG; }
+    StreamScope scope(S->st);   /* buffers (re)allocated here belong to the model's stream */
     const int B = std::min(65536, ((n + TILE - 1) / TILE) * TILE);
     GF_TRY(ensure_batch(*S, B));
     std::vector<float> xb((size_t)B * 6, 0.0f);
@@ -585,6 +588,7 @@ int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, flo
 /* raw ensemble logits for n inputs (n a multiple of 64): [E][n][128] */
 int32_t gf_sg_forward(gf_sg_model *S, const float *x, int32_t n, float *logits) {
     if (!S || !x || !logits || n <= 0 || n % TILE) { set_error("gf_sg_forward: invalid arguments"); return GF_ERR_ARG; }
+    StreamScope scope(S->st);   /* buffers (re)allocated here belong to the model's stream */
     GF_TRY(ensure_batch(*S, n));
     GF_TRY(load_batch(*S, x, nullptr, nullptr, n));
     GF_TRY(forward(*S, n));

@@ -1,0 +1,135 @@
+"""The benchmark configurations at their configured sizes on the GPU:
+
+  large single population  N = 1e5, L = 5e8, gamma = -10, h = 0.5, 5,000
+                           generations from equilibrium, SFS n = 100, 1000
+  three populations        N0 = 2e4, L = 2e8, growth to 2e5 by generation
+                           500, a bottleneck in [500, 600), F = 0.1 in the
+                           third, sine-wave selection, preserve every 100,
+                           2,000 generations, SFS n = 50 per population
+  batched sweep            4,096 single-population simulations (N
+                           log-uniform on [1e3, 1e5], 4N/10 generations)
+
+Each runs in full on the GPU; the oracle checks a prefix of the run that
+it finishes in seconds (bit for bit), and size-independent properties
+check the rest."""
+import numpy as np
+import pytest
+
+from model_reference import mse_lambda_ref
+from parallelpopgen_amd import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b, what):
+    assert len(a["samples"]) == len(b["samples"]), what
+    for i, (x, y) in enumerate(zip(a["samples"], b["samples"])):
+        assert x["generation"] == y["generation"] and x["num_mutations"] == y["num_mutations"], (what, i)
+        assert np.array_equal(x["freq"].view(np.uint32), y["freq"].view(np.uint32)), (what, i)
+    assert np.array_equal(a["ids"], b["ids"]), what
+    assert np.array_equal(a["live"], b["live"]), what
+
+
+@pytest.fixture
+def oracle16(oracle):
+    oracle.set_threads(16)
+    yield oracle
+    oracle.set_threads(1)
+
+
+def test_plain_large_prefix_matches_oracle(gpu, oracle16):
+    """full L = 5e8 from equilibrium (2.0e6 live mutations, 1e5 new per
+    generation): the first 300 generations equal the oracle's bit for bit,
+    every generation's live count included"""
+    d = S.plain_large(G=300)
+    print("plain_large L=5e8 G=300", flush=True)
+    g = gpu.run(d)
+    o = oracle16.run(d)
+    assert g["samples"][-1]["num_mutations"] > 1_800_000
+    assert_same(g, o, "plain_large prefix")
+
+
+def test_plain_large_full_run_properties(gpu, oracle16):
+    """all 5,000 generations: the live count stays at the equilibrium the
+    reference's model predicts (sum of the equilibrium class expectations,
+    go_fish_impl.cuh:20-71, in float64: 2.003e6), and the final sample
+    SFS at n = 100 and 1000 (binned on the device) equals the oracle's SFS
+    of the same final state bit for bit and sums to the number of sites"""
+    d = S.plain_large()
+    print("plain_large L=5e8 G=5000", flush=True)
+    r = gpu.run(d, final_sfs=[(0, 100), (0, 1000)])
+    s = S.selection_from_gamma(-10.0, 100000, 0.0)
+    expect = mse_lambda_ref(100000, 0.0, 0.5, 1e-9, 5e8, lambda x: np.full(np.shape(x), float(s))).sum()
+    live = r["live"][1:].astype(np.float64)
+    assert abs(live[1000:].mean() / expect - 1) < 0.01, (live[1000:].mean(), expect)
+    assert abs(live.max() / expect - 1) < 0.03 and abs(live.min() / expect - 1) < 0.03
+    last = r["samples"][-1]
+    row = last["freq"][0]
+    assert last["num_mutations"] == len(row) and last["generation"] == 5000
+    for n in (100, 1000):
+        dev = r["final_sfs"][(0, n)]
+        ref = oracle16.site_frequency_spectrum(row, n, d.num_sites)
+        assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32)), n
+        # every site is in exactly one bin: the float sum is L up to the
+        # rounding of bin 0 (about 5e8, spacing 32) and the fixed-point
+        # truncation of the others
+        assert abs(dev.astype(np.float64).sum() - d.num_sites) <= 64, n
+
+
+def test_three_population_prefix_matches_oracle(gpu, oracle16):
+    """full N0 = 2e4, L = 2e8: 650 generations, spanning the end of growth
+    and the bottleneck's onset (500) and end (600) and six preserve
+    generations, equal the oracle's bit for bit"""
+    d = S.three_population(G=650)
+    print("three_population full size G=650", flush=True)
+    g = gpu.run(d)
+    o = oracle16.run(d)
+    assert_same(g, o, "three_population prefix")
+    ids = g["ids"]["origin_threadID"]
+    assert np.sum(ids < 0) > 0   # preserved mutations carried to the end
+
+
+def test_three_population_full_run_properties(gpu, oracle16):
+    """all 2,000 generations with preserve every 100: per-population SFS at
+    n = 50 on the device equal to the oracle's of the same state; preserved
+    mutations are never dropped (every mutation segregating at a preserve
+    generation is still in the final state); population 0's ten-fold growth
+    shows in its number of segregating sites"""
+    d = S.three_population(sample_every=500)
+    print("three_population full size G=2000", flush=True)
+    r = gpu.run(d, final_sfs=[(0, 50), (1, 50), (2, 50)])
+    samples = r["samples"]
+    assert [x["generation"] for x in samples] == [500, 1000, 1500, 2000]
+    last = samples[-1]
+    for p in range(3):
+        dev = r["final_sfs"][(p, 50)]
+        ref = oracle16.site_frequency_spectrum(last["freq"][p], 50, d.num_sites)
+        assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32)), p
+    # samples are preserve points too: every mutation present at a sample is
+    # kept (with its ID negated) and stays in the same order
+    M500 = samples[0]["num_mutations"]
+    assert np.all(r["ids"]["origin_threadID"][:M500] < 0)
+    seg = lambda x, p: int(np.sum((x["freq"][p] > 0) & (x["freq"][p] < 1)))
+    assert seg(samples[1], 0) > 3 * seg(samples[1], 2)
+
+
+def test_batched_sweep_full_size_bit_identical_to_plain(gpu):
+    """the 4,096-simulation sweep in one batch; one simulation from each of
+    40 equal strata of expected work (live sites x generations) run through
+    gf_run has the batch's SFS and final mutation count bit for bit"""
+    sims = S.parameter_sweep(4096)
+    print("batched sweep 4096", flush=True)
+    b = gpu.batch_run(sims, compact_interval=16, sfs_n=50)
+    assert b["sfs"].shape == (4096, 50) and np.all(np.isfinite(b["sfs"]))
+    Nc = np.floor(2.0 * sims["N"] / (1.0 + sims["F"]))
+    work = sims["num_sites"] * 2e-9 * Nc * S.harmonic(Nc - 1) * sims["num_generations"]
+    strata = np.array_split(np.argsort(work), 40)
+    rng = np.random.default_rng(5)
+    for st in strata:
+        k = int(rng.choice(st))
+        d = S.sweep_plain_desc(sims[k])
+        r = gpu.run(d)
+        last = r["samples"][-1]
+        assert b["num_mutations"][k] == last["num_mutations"], k
+        sfs = gpu.site_frequency_spectrum(last["freq"][0], 50, d.num_sites)
+        assert np.array_equal(b["sfs"][k].view(np.uint32), sfs.view(np.uint32)), k

@@ -99,3 +99,27 @@ def test_reused_device_memory_batched(gpu):
     for s, ((fa, ia), (fb, ib)) in enumerate(zip(a["states"], b["states"])):
         assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32)), s
         assert np.array_equal(ia, ib), s
+
+
+def test_release_device_cache(gpu, oracle):
+    """gf_release_device_cache returns the kept blocks to the device; the
+    next run allocates afresh and gives the same result.  A result's own
+    device buffers (the final state kept for device-side spectra) outlive
+    the run's stream and stay valid across the release."""
+    d = S.plain_small(G=80)
+    a = gpu.run(d)
+    lib = gpu.load()
+    h = gpu.C.c_void_p()
+    gpu.check(lib.gf_run(gpu.C.byref(d.desc), gpu.C.byref(h)))
+    try:
+        assert lib.gf_release_device_cache(-1) == 0
+        last = lib.gf_result_num_samples(h) - 1
+        out = np.empty(50, np.float32)
+        gpu.check(lib.gf_result_site_frequency_spectrum(h, last, 0, 50, float(d.num_sites),
+                                                         out.ctypes.data_as(gpu.C.c_void_p)))
+    finally:
+        lib.gf_result_free(h)
+    want = oracle.site_frequency_spectrum(a["samples"][-1]["freq"][0], 50, d.num_sites)
+    assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+    assert lib.gf_release_device_cache(-1) == 0
+    assert_same(gpu.run(d), a, "after releasing the cache")
