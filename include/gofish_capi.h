@@ -316,11 +316,6 @@ int32_t gf_set_kernel_timing(int32_t enable);
  * 1.2e6, 10 beyond; several: 64 up to 2e6 slots, 20 beyond, a launch
  * ending at each compaction generation either way; batches: 16). */
 int32_t gf_set_generations_per_launch(int32_t k);
-/* Draw-path regrouping of the drift kernel: at the start of a launch, the
- * slots of a tile whose count is within `threshold` of 0 or N (Poisson tails
- * of the approximate binomial) are gathered into the same waves, and the
- * rest into others; results are identical with any threshold (0: off). */
-int32_t gf_set_draw_regroup(float threshold);
 int32_t gf_device_count(int32_t *out);
 
 #ifdef __cplusplus

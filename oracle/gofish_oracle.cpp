@@ -195,6 +195,7 @@ static float pow_f(float a, float b) {
 #include "../parallelpopgen_amd/csrc/gf_ncdf_table.h"
 
 static float normcdfinv_f(float p) {
+    p = p > 0x1p-33f ? p : 0x1p-33f;   /* below the table's domain, and NaN: 2^-33 */
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
     const float s = q < 0.0f ? p : 1.0f - p;

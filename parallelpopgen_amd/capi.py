@@ -78,7 +78,6 @@ SIGNATURES = {
     "gf_result_free": (None, [C.c_void_p]),
     "gf_check_sim": (C.c_int32, [C.POINTER(gf_sim_desc)]),
     "gf_release_device_cache": (C.c_int32, [C.c_int32]),
-    "gf_set_draw_regroup": (C.c_int32, [C.c_float]),
     "gf_population_histogram": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_batch_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -312,10 +311,6 @@ def set_kernel_timing(on):
 
 def set_generations_per_launch(k):
     check(load().gf_set_generations_per_launch(int(k)))
-
-
-def set_draw_regroup(threshold):
-    check(load().gf_set_draw_regroup(float(threshold)))
 
 
 MATH_OPS = dict(philox=0, u01=1, exp=2, log=3, normcdfinv=4, binom_draw=5, pois_draw=6, sqrt=7, div=8,

@@ -371,7 +371,6 @@ struct FusedLaunch {
     int dense;                    /* input is dense: compacted slot li is slot li */
     int compact;
     int preserve;
-    float regroup;                /* > 0: regroup tiles that hold carried-over slots by draw path (below) */
 };
 
 /* Slot of compacted index li (< M_in) in a fragmented input, for every lane
@@ -418,16 +417,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     __shared__ unsigned long long sh_live[64];
     __shared__ int sh_diff[KMAX + 1];
     __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
-    /* regrouping: per-wave class counts, the staged slot states, and the
-     * keep flags / output positions in the tile's original order */
-    constexpr bool kRegroup = P <= 3;          /* beyond 3 populations its LDS would cost occupancy */
-    constexpr int RG = kRegroup ? WG : 1;
-    __shared__ int sh_cls[WG / 64][2];
-    __shared__ float sh_x[P][RG];
-    __shared__ int4 sh_id[RG];
-    __shared__ int sh_born[RG];
-    __shared__ unsigned short sh_orig[RG];
-    __shared__ unsigned short sh_pos[RG];
     load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
     if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
     if (threadIdx.x <= KMAX) sh_diff[threadIdx.x] = 0;
@@ -493,70 +482,17 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             }
         }
         const auto *crows = as_const(rows);
-        /* Regrouping by draw path (tiles holding slots carried over from the
-         * previous launch; a tile of new slots is grouped by birth already):
-         * slots whose count in some population is within `regroup` of 0 or
-         * of N (the Poisson tails of the approximate binomial), and slots
-         * born in this launch, go to the front of the tile, the rest after
-         * them, empty lanes last, so that most waves take one draw path.
-         * Each slot's arithmetic is unchanged (its random stream is keyed by
-         * its ID); the output keeps the original order. */
-        const bool regroup = kRegroup && F.regroup > 0.0f && tile * WG < M0;
-        int orig = (int)threadIdx.x;
-        bool ok = valid;
-        if (regroup) {
-            const auto &R0 = crows[0];
-            bool lo = born >= 0;
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-                if (R0.Nc[p] > 0) {
-                    const float c = x[p] * R0.fN[p];
-                    lo = lo || c <= F.regroup || c >= R0.fN[p] - F.regroup;
-                }
-            }
-            const int cls = valid ? (lo ? 0 : 1) : 2;
-            const uint64_t b0 = __ballot(cls == 0), b1 = __ballot(cls == 1);
-            if (lane == 0) { sh_cls[w][0] = __popcll(b0); sh_cls[w][1] = __popcll(b1); }
-            __syncthreads();
-            int n0 = 0, n1 = 0, base0 = 0, base1 = 0;
-#pragma unroll
-            for (int k = 0; k < WG / 64; k++) {
-                if (k < w) { base0 += sh_cls[k][0]; base1 += sh_cls[k][1]; }
-                n0 += sh_cls[k][0];
-                n1 += sh_cls[k][1];
-            }
-            const uint64_t mine = cls == 0 ? b0 : (cls == 1 ? b1 : ~(b0 | b1));
-            const int r = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
-            const int dst = cls == 0 ? base0 + r : (cls == 1 ? n0 + base1 + r : n0 + n1 + (w * 64 - base0 - base1) + r);
-#pragma unroll
-            for (int p = 0; p < P; p++) sh_x[p][dst] = x[p];
-            sh_id[dst] = make_int4(id.origin_generation, id.origin_population, id.origin_threadID, id.reserved);
-            sh_born[dst] = born;
-            sh_orig[dst] = (unsigned short)threadIdx.x;
-            __syncthreads();
-#pragma unroll
-            for (int p = 0; p < P; p++) x[p] = sh_x[p][threadIdx.x];
-            const int4 v = sh_id[threadIdx.x];
-            id.origin_generation = v.x; id.origin_population = v.y; id.origin_threadID = v.z; id.reserved = v.w;
-            born = sh_born[threadIdx.x];
-            orig = sh_orig[threadIdx.x];
-            ok = tile * WG + orig < Mend;
-            ogen = id.origin_generation;
-            opop = id.origin_population;
-            preserved = born < 0 && id.origin_threadID < 0;
-            tid = (uint32_t)(id.origin_threadID < 0 ? -id.origin_threadID : id.origin_threadID);
-        }
         u32x4 rc[P];
         bool keep = false;
         int kfirst = KMAX, klast = -1;
         for (int t = 0; t < S.K; t++) {
             const auto &R = crows[t];
             const int g = S.g0 + 1 + t;
-            if (ok && born < t) {
+            if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
                 generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel, sh_ncdf);
             }
-            keep = ok && born <= t && keep_rule<P>(x, R.present_mask, preserved);
+            keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             if constexpr (P == 1) {
                 /* one population: a slot is kept over one run of generations
                  * (lost and fixed are absorbing), recorded by its ends */
@@ -575,27 +511,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             }
         }
         /* without compaction every slot of the block stays */
-        if (!F.compact) keep = ok;
-        /* tile-local stable packing: position among the kept slots in the
-         * tile's original order (a regrouped tile ranks through LDS) */
-        bool kflag = keep;
-        if (regroup) {
-            sh_pos[orig] = keep ? 1 : 0;
-            __syncthreads();
-            kflag = sh_pos[threadIdx.x] != 0;
-        }
-        const uint64_t kb = __ballot(kflag);
+        if (!F.compact) keep = valid;
+        /* tile-local stable packing */
+        const uint64_t kb = __ballot(keep);
         if (lane == 0) sh_wcount[w] = (unsigned int)__popcll(kb);
         __syncthreads();
         unsigned int pos = 0;
         for (int k = 0; k < w; k++) pos += sh_wcount[k];
         pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(kb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)kb, 0u));
-        if (regroup) {
-            /* every flag was read before the barrier above */
-            sh_pos[threadIdx.x] = (unsigned short)pos;
-            __syncthreads();
-            pos = sh_pos[orig];
-        }
         if (keep) {
             const int64_t o = (int64_t)tile * WG + pos;
 #pragma unroll

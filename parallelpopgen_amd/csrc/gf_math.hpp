@@ -160,7 +160,7 @@ GF_HD float sqrt_pos(float x) {
 }
 
 /* Inverse standard normal CDF for p in [2^-33, 1] (the range of u01; p is
- * clamped to <= 1 - 2^-24 so the result stays finite), from a segment
+ * clamped into [2^-33, 1 - 2^-24], NaN to 2^-33), from a segment
  * table (gf_ncdf_table.h, fitted by parallelpopgen_amd/csrc/fit_normcdfinv.py):
  * with s = min(p, 1 - p) in [2^-33, 1/2], the float bits of s pick one of
  * 16 segments per binade (biased exponent and top 4 mantissa bits; the
@@ -180,6 +180,10 @@ GF_HD const NcdfSeg *ncdf_table() { return reinterpret_cast<const NcdfSeg *>(kNc
 
 GF_HD float normcdfinv_tab(const NcdfSeg *tab, float p) {
     GF_STRICT_FP
+    /* the table's domain [2^-33, 1 - 2^-24] (u01's range, its top end
+     * clamped): anything below 2^-33, and NaN, is taken as 2^-33, so the
+     * segment index stays in the table for every input */
+    p = p > 0x1p-33f ? p : 0x1p-33f;
     if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
     const float s = q < 0.0f ? p : 1.0f - p;        /* [2^-33, 1/2], exact */

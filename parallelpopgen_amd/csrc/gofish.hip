@@ -59,7 +59,6 @@ const std::string &get_error() { return t_err; }
 static int g_timing = 0;
 int timing_stride() { return g_timing; }
 static int g_kpl = 0;    /* generations per advance launch (1..KMAX); 0: the caller's default */
-static float g_regroup = 0.0f;   /* drift kernel's draw-path regrouping threshold; 0: off */
 int generations_per_launch(int dflt) { return g_kpl > 0 ? g_kpl : dflt; }
 
 int select_device(int device) {
@@ -550,7 +549,6 @@ struct PlainRun {
         F.dense = frag ? 0 : 1;
         F.compact = compact ? 1 : 0;
         F.preserve = preserve ? 1 : 0;
-        F.regroup = g_regroup;
         SegLaunch<P> sl{};
         sl.rows = d_rows.p + t_first;
         sl.live = d_live.p;
@@ -784,12 +782,6 @@ int32_t gf_set_kernel_timing(int32_t enable) {
 int32_t gf_set_generations_per_launch(int32_t k) {
     if (k < 0 || k > gf::KMAX) { gf::set_error("generations per launch must be in [1, 64] (0: default)"); return GF_ERR_ARG; }
     gf::g_kpl = k;
-    return GF_OK;
-}
-
-int32_t gf_set_draw_regroup(float threshold) {
-    if (!(threshold >= 0.0f)) { gf::set_error("draw regrouping threshold must be >= 0"); return GF_ERR_ARG; }
-    gf::g_regroup = threshold;
     return GF_OK;
 }
 

@@ -272,25 +272,3 @@ def test_population_histogram_bin_ranges(gpu, oracle, Nc):
     h_o = oracle.population_frequency_histogram(row, Nc, 5e8)
     assert h_g.shape == (Nc,)
     assert np.array_equal(h_g, h_o)
-
-
-@pytest.mark.parametrize("threshold", [0.0, 4.0, 12.0, 1e9])
-def test_draw_regrouping_invariance(gpu, oracle, threshold):
-    """regrouping a tile's slots by draw path changes which lane draws for
-    which mutation, never the draws: one population (selected, launches of
-    64 and of 10 generations) and two populations (split, migration,
-    extinction) equal the oracle with every threshold"""
-    cases = [("plain_large L=2e6", S.plain_large(L=2e6, G=120), 0),
-             ("plain_small kpl 10", S.plain_small(G=150, compact_interval=20), 10),
-             ("split", split_scenario(split=101), 0)]
-    gpu.set_draw_regroup(threshold)
-    try:
-        for what, d, kpl in cases:
-            gpu.set_generations_per_launch(kpl)
-            try:
-                g = gpu.run(d)
-            finally:
-                gpu.set_generations_per_launch(0)
-            assert_same(g, oracle.run(d), "%s regroup=%g" % (what, threshold))
-    finally:
-        gpu.set_draw_regroup(0.0)

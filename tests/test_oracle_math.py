@@ -119,6 +119,17 @@ def test_normcdfinv_error_bound(oracle):
     assert oracle.normcdfinv(np.array([0.5], np.float32))[0] == 0.0
 
 
+def test_normcdfinv_outside_its_domain(oracle):
+    # inputs below u01's smallest value 2^-33 (the table starts there), 0,
+    # negatives and NaN are taken as 2^-33; above 1 - 2^-24 as 1 - 2^-24
+    lo = oracle.normcdfinv(np.array([2.0 ** -33], np.float32))[0]
+    x = np.array([1.03e-10, 1e-20, 1e-45, 0.0, -0.0, -1.0, -np.inf, np.nan], np.float32)
+    z = oracle.normcdfinv(x)
+    assert np.all(z == lo) and abs(lo - sp.ndtri(2.0 ** -33)) < 1e-5
+    hi = oracle.normcdfinv(np.array([1 - 2.0 ** -24], np.float32))[0]
+    assert np.all(oracle.normcdfinv(np.array([1.0, 1.5, np.inf], np.float32)) == hi)
+
+
 def test_normcdfinv_monotone(oracle):
     # non-decreasing to within one ulp of the result over consecutive u01
     # values (consecutive inputs move the result by a fraction of an ulp in
