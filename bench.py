@@ -153,7 +153,7 @@ def cpu_baseline(scenarios, G_gpu, want_gens):
 def bench_batched(a, capi, scenarios):
     import numpy as np
     sims = scenarios.parameter_sweep(a.sims)
-    interval = 16
+    interval = 32
     capi.batch_run(sims[:64], compact_interval=interval, sfs_n=50)     # warm-up
     times = []
     for _ in range(max(1, a.steps)):

@@ -227,8 +227,11 @@ typedef struct gf_batch_result gf_batch_result;
 /* Runs every simulation from mutation-selection equilibrium to its own final
  * generation, one population each, and reports the final SFS of a sample of
  * sfs_n chromosomes per simulation.  Per-simulation results are identical
- * to gf_run on the equivalent gf_sim_desc.  Generations advance in blocks of
- * min(compact_interval, 64, gf_set_generations_per_launch) per launch. */
+ * to gf_run on the equivalent gf_sim_desc with any compaction interval > 0.
+ * Generations advance in blocks (one captured graph replay each) of at most
+ * min(compact_interval, 32, gf_set_generations_per_launch) generations, the
+ * length chosen by the library's autotuner; every block compacts, so
+ * compact_interval must be > 0 (GF_ERR_ARG otherwise). */
 int32_t gf_batch_run(const gf_batch_sim *sims, int32_t num_sims,
                      int32_t compact_interval, int32_t sfs_n,
                      int32_t keep_states, int32_t device, gf_batch_result **out);
@@ -240,6 +243,10 @@ int32_t gf_batch_num_mutations(const gf_batch_result *r, int32_t *out /*[num_sim
 int32_t gf_batch_sim_state(const gf_batch_result *r, int32_t sim, float *freq, gf_mutid *ids);
 int32_t gf_batch_live_mutation_generations(const gf_batch_result *r, int64_t *out);
 int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out);
+/* the block length the autotuner settled on, the number of graph replays
+ * and the number of re-layouts (simulations that outgrew their tiles) */
+int32_t gf_batch_tuning(const gf_batch_result *r, int32_t *generations_per_block, int32_t *blocks,
+                        int32_t *relayouts);
 void    gf_batch_free(gf_batch_result *r);
 
 /* ----- learned surrogate of the sample SFS (no reference counterpart) -----

@@ -87,6 +87,7 @@ SIGNATURES = {
     "gf_batch_sim_state": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "gf_batch_live_mutation_generations": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     "gf_batch_timing": (C.c_int32, [C.c_void_p, C.POINTER(gf_timing)]),
+    "gf_batch_tuning": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gf_batch_free": (None, [C.c_void_p]),
     "gf_run_with_selection": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gf_result_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p]),
@@ -370,6 +371,9 @@ def batch_run(sims, compact_interval=35, sfs_n=50, keep_states=False, device=-1,
             t = gf_timing()
             lib.gf_batch_timing(h, C.byref(t))
             out["timing"] = {k: getattr(t, k) for k, _ in gf_timing._fields_}
+        k, nb, nr = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib.gf_batch_tuning(h, C.byref(k), C.byref(nb), C.byref(nr)))
+        out["tuning"] = dict(generations_per_block=k.value, blocks=nb.value, relayouts=nr.value)
     finally:
         lib.gf_batch_free(h)
     return out

@@ -1,36 +1,57 @@
 /*
  * batch.hip -- batched simulation: thousands of independent single-
- * population simulations advanced together, each launch covering every
- * live simulation (no reference counterpart; it replaces the back-to-back
- * run_sim + site_frequency_spectrum loop of examples/example_dadi/run.cu:
- * 83-94, which recreates streams and buffers and synchronises the host
- * several times per simulation).
+ * population simulations advanced together (no reference counterpart; it
+ * replaces the back-to-back run_sim + site_frequency_spectrum loop of
+ * examples/example_dadi/run.cu:83-94, which recreates streams and buffers
+ * and synchronises the host several times per simulation).
  *
- * All simulations share one structure-of-arrays buffer; simulation s owns
- * the slot range [base_s, base_s + cap_s), 256-slot aligned, so every
- * 256-slot tile belongs to exactly one simulation and per-simulation
- * parameters are wave-uniform scalar loads.  The per-generation kernels are
- * the plain path's (k_advance, k_flag, k_scan_tiles, k_scatter, k_mse_*,
- * k_sfs) driven through per-simulation offset tables, and the RNG is keyed
- * by (seed, generation, mutation ID), so every simulation's result is bit
- * for bit the result gf_run gives for it alone.
+ * Layout.  Two state buffers (frequencies + mutation IDs); simulation s owns
+ * the tiles [t0_s, t0_s + nt_s) of both (256 slots per tile), so every tile
+ * belongs to one simulation and its parameters are wave-uniform scalar
+ * loads.  Each simulation advances on its own: its generation counter, the
+ * buffer holding its current state and that state's layout (dense, or
+ * fragmented: tile j holds its survivors packed at its start, described by
+ * the prefix pre_s and the source-tile table src0_s) live in a device
+ * array of BSim records.
  *
- * Per block of up to min(compact_interval, 64) generations the host:
- *   1. flags survivors of the current state (segmented) and draws the next
- *      block's new-mutation counts on the device,
- *   2. reads back the per-simulation survivor and new-mutation totals (the
- *      only synchronisation of the block),
- *   3. scatters survivors into a fresh layout sized for the block,
- *   4. finalises simulations that reached their last generation (SFS),
- *   5. launches one k_advance over all live simulations.
+ * One block (up to 32 generations) is four kernels over every simulation,
+ * with no host round trip:
+ *   k_bt_rows     per simulation: the block's generation rows and new-
+ *                 mutation counts; a simulation whose survivors and new
+ *                 mutations would not fit its tiles stalls (its state stays)
+ *   k_bt_advance  the drift kernel with tile-local compaction
+ *                 (migration_selection_drift + add_new_mutations +
+ *                 compaction, as k_advance_fused): reads the current
+ *                 buffer, writes each tile's survivors packed into the other
+ *   k_bt_scan     per simulation: prefix of its tiles' survivor counts (the
+ *                 next block's gather table), new slot count, generation
+ *   k_bt_status   active / stalled counts and the live mutation-generations
+ *                 into pinned host memory
+ * The block is captured once in a hipGraph and replayed; the host looks at
+ * the status every few replays.  Stalled simulations get larger regions
+ * (the host re-lays out the live ones, densely, into new buffers, which also
+ * drops finished simulations once they hold most of the tiles).  Finished
+ * simulations are finalised (SFS of n chromosomes, optionally the state)
+ * from wherever their last block left them.
+ *
+ * Block length: a small online autotuner measures the batch's own
+ * throughput (live mutation-generations per second) with several lengths
+ * on the first replays and keeps the best for the rest of the batch and
+ * for later batches of the same shape.  Results do not depend on it: draws
+ * are keyed by (seed, generation, mutation ID), so every simulation's
+ * result is bit for bit the result gf_run gives for it alone.
  */
 #include "gf_kernels.hpp"
 #include "gf_runtime.hpp"
 #include "gf_spectrum.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 struct gf_batch_result {
@@ -41,47 +62,448 @@ struct gf_batch_result {
     std::vector<std::vector<gf_mutid>> ids;
     int64_t live_mutation_generations = 0;
     gf_timing timing{};
+    int32_t generations_per_block = 0;       /* chosen by the autotuner (or the caller's cap) */
+    int32_t blocks = 0;                      /* graph replays */
+    int32_t relayouts = 0;
 };
 
 namespace gf {
 
-struct BatchSimDev {
+constexpr int BK_MAX = 32;                   /* longest block the autotuner tries */
+enum { BS_ACTIVE = 0, BS_DONE = 1, BS_STALLED = 2 };
+
+/* one simulation of a batch, device-resident */
+struct alignas(16) BSim {
+    /* model */
+    double invN;
     int Nc, N, G, neutral;
     float F, h, mu, s, L, c1, c2, newfreq, fN;
     uint32_t k0, k1;
-    int pad;
+    /* layout: tiles [t0, t0 + nt) of both buffers, prefix entries [p0, p0 + nt] */
+    int t0, nt, p0;
+    /* progress */
+    int g;        /* generations done */
+    int M;        /* slots (all segregating) in the current buffer */
+    int T;        /* tiles of the current fragmented state; -1: dense */
+    int cur;      /* buffer of the current state */
+    int status;   /* BS_* */
+    /* this block (k_bt_rows) */
+    int Kb, nnew, tiles;
 };
 
-/* new-mutation counts and model rows of the next block for every active
- * simulation (one thread per simulation; sequential cumulative counts) */
-static __global__ void k_batch_rows(const BatchSimDev *__restrict__ sims, const int *__restrict__ active, int nact,
-                                    int g0, int K, GenRow<1> *__restrict__ rows, int *__restrict__ tot) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nact) return;
-    const BatchSimDev S = sims[active[a]];
-    const int Ks = min(K, S.G - g0);
-    int cum = 0;
-    for (int t = 0; t < Ks; t++) {
-        const int g = g0 + 1 + t;
-        GenRow<1> r;
-        r.Nc[0] = S.Nc;
-        r.fN[0] = S.fN;
-        r.invN[0] = S.Nc > 0 ? 1.0 / (double)S.fN : 0.0;
-        r.sa[0] = 0.0f;
-        r.sb[0] = S.s;
-        r.c1[0] = S.c1;
-        r.c2[0] = S.c2;
-        r.mig[0][0] = 1.0f;
-        r.newfreq[0] = S.newfreq;
-        r.newcum[0] = cum;
-        cum += new_mutation_count(S.Nc, S.mu, S.L, S.k0, S.k1, g, 0);
-        r.newcum[1] = cum;
-        r.present_mask = S.N > 0 ? 1 : 0;
-        r.neutral_mask = S.neutral ? 1 : 0;
-        rows[(size_t)a * K + t] = r;
-    }
-    tot[a] = cum;
+/* device pointers of a layout (kernel argument, fixed while a graph lives) */
+struct BBufs {
+    float *freq[2];
+    gf_mutid *ids[2];
+    int *pre[2];                  /* per simulation nt + 1 entries from p0 */
+    int *src0[2];                 /* per simulation nt entries from t0 */
+    unsigned int *cnt;            /* survivors per output tile */
+    GenRow<1> *rows;              /* [S][BK_MAX] */
+    const int *tile_sim;          /* tile -> simulation */
+    unsigned long long *live;     /* 64 partial sums of live mutation-generations */
+    int *K;                       /* block length (device word: the tuner changes it between replays) */
+    int ntiles, S;
+};
+
+/* a BSim read through scalar loads (uniform per tile) */
+__device__ __forceinline__ BSim load_bsim(const BSim *p) {
+    const auto &c = *as_const(p);
+    BSim b;
+    b.invN = c.invN; b.Nc = c.Nc; b.N = c.N; b.G = c.G; b.neutral = c.neutral;
+    b.F = c.F; b.h = c.h; b.mu = c.mu; b.s = c.s; b.L = c.L; b.c1 = c.c1; b.c2 = c.c2;
+    b.newfreq = c.newfreq; b.fN = c.fN; b.k0 = c.k0; b.k1 = c.k1;
+    b.t0 = c.t0; b.nt = c.nt; b.p0 = c.p0; b.g = c.g; b.M = c.M; b.T = c.T; b.cur = c.cur;
+    b.status = c.status; b.Kb = c.Kb; b.nnew = c.nnew; b.tiles = c.tiles;
+    return b;
 }
+
+/* ---- 1. rows and new-mutation counts of the next block, one thread per simulation ---- */
+static __global__ void k_bt_rows(BSim *__restrict__ sims, BBufs B) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B.S) return;
+    BSim &b = sims[s];
+    b.Kb = 0;
+    b.tiles = 0;
+    if (b.status != BS_ACTIVE || b.g >= b.G) return;
+    const int K = min(*B.K, b.G - b.g);
+    GenRow<1> *rows = B.rows + (size_t)s * BK_MAX;
+    int cum = 0;
+    for (int t = 0; t < K; t++) {
+        const int g = b.g + 1 + t;
+        GenRow<1> r;
+        r.Nc[0] = b.Nc;
+        r.fN[0] = b.fN;
+        r.invN[0] = b.invN;
+        r.sa[0] = 0.0f;
+        r.sb[0] = b.s;
+        r.c1[0] = b.c1;
+        r.c2[0] = b.c2;
+        r.mig[0][0] = 1.0f;
+        r.newfreq[0] = b.newfreq;
+        r.newcum[0] = cum;
+        cum += new_mutation_count(b.Nc, b.mu, b.L, b.k0, b.k1, g, 0);
+        r.newcum[1] = cum;
+        r.present_mask = b.N > 0 ? 1 : 0;
+        r.neutral_mask = b.neutral ? 1 : 0;
+        rows[t] = r;
+    }
+    const int64_t need = (int64_t)b.M + cum;
+    const int tiles = (int)((need + WG - 1) / WG);
+    if (tiles > b.nt) {   /* does not fit its region: wait for the host */
+        b.status = BS_STALLED;
+        return;
+    }
+    b.Kb = K;
+    b.nnew = cum;
+    b.tiles = tiles;
+}
+
+/* ---- 2. drift + new mutations + tile-local compaction of every simulation's block ---- */
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_bt_advance(const BSim *__restrict__ sims,
+                                                                                          const BBufs B) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    __shared__ unsigned int sh_wcount[WG / 64];
+    __shared__ int sh_pre[WG + 1];
+    __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
+    load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
+    uint64_t live = 0;          /* this thread's slots' live mutation-generations */
+    for (int gt = blockIdx.x;; gt += gridDim.x) {
+        __syncthreads();
+        if (gt >= B.ntiles) break;
+        const int s = *as_const(B.tile_sim + gt);
+        const BSim S = load_bsim(sims + s);
+        const int j = gt - S.t0;
+        if (j >= S.tiles) continue;
+        /* the two buffers by selects (indexing the argument's arrays with
+         * a run-time value would put them in scratch memory) */
+        const bool c1 = S.cur != 0;
+        const float *freq_in = c1 ? B.freq[1] : B.freq[0];
+        float *freq_out = c1 ? B.freq[0] : B.freq[1];
+        const gf_mutid *ids_in = c1 ? B.ids[1] : B.ids[0];
+        gf_mutid *ids_out = c1 ? B.ids[0] : B.ids[1];
+        const int *pre_in = (c1 ? B.pre[1] : B.pre[0]) + S.p0;
+        const int *src0_in = (c1 ? B.src0[1] : B.src0[0]) + S.t0;
+        const int64_t base = (int64_t)S.t0 * WG;
+        const int M0 = S.M, Mend = S.M + S.nnew;
+        const bool dense = S.T < 0;
+        const int li = j * WG + (int)threadIdx.x;
+        const bool valid = li < Mend;
+        const GenRow<1> *rows = B.rows + (size_t)s * BK_MAX;
+        int s0 = 0;
+        if (!dense && j * WG < M0) s0 = load_frag_window(j, src0_in, pre_in, S.T, sh_pre);
+        float x = 0.0f;
+        int ogen = 0, opop = 0, born = -1;
+        uint32_t tid = 0;
+        gf_mutid id;
+        id.origin_generation = 0; id.origin_population = 0; id.origin_threadID = 0; id.reserved = 0;
+        if (valid) {
+            if (li < M0) {
+                const int64_t src = base + (dense ? (int64_t)li : frag_slot(li, s0, sh_pre, pre_in, S.T));
+                x = freq_in[src];
+                id = ids_in[src];
+                ogen = id.origin_generation;
+                opop = id.origin_population;
+                tid = (uint32_t)(id.origin_threadID < 0 ? -id.origin_threadID : id.origin_threadID);
+            } else {
+                const int rel = li - M0;
+                int tb, pb;
+                find_birth<1>(rows, S.Kb, rel, tb, pb);
+                born = tb;
+                ogen = S.g + 1 + tb;
+                opop = pb;
+                tid = (uint32_t)(rel - rows[tb].newcum[pb] + 1);
+                x = rows[tb].newfreq[0];
+                id.origin_generation = ogen;
+                id.origin_population = opop;
+                id.origin_threadID = (int32_t)tid;
+                id.reserved = 0;
+            }
+        }
+        const auto *crows = as_const(rows);
+        u32x4 rc[1];
+        bool keep = false;
+        int kfirst = KMAX, klast = -1;
+        for (int t = 0; t < S.Kb; t++) {
+            const auto &R = crows[t];
+            const int g = S.g + 1 + t;
+            if (valid && born < t) {
+                const bool refresh = ((g & 3) == 0) || (t == born + 1);
+                generation_update<1>(&x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf);
+            }
+            keep = valid && born <= t && keep_rule<1>(&x, R.present_mask, false);
+            /* one population: kept over one run of generations (lost and
+             * fixed are absorbing) */
+            if (keep) { kfirst = t < kfirst ? t : kfirst; klast = t; }
+        }
+        if (klast >= 0) live += (uint64_t)(klast - kfirst + 1);
+        /* tile-local stable packing into the other buffer */
+        const uint64_t kb = __ballot(keep);
+        if (lane == 0) sh_wcount[w] = (unsigned int)__popcll(kb);
+        __syncthreads();
+        if (keep) {
+            unsigned int pos = 0;
+            for (int k = 0; k < w; k++) pos += sh_wcount[k];
+            pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(kb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)kb, 0u));
+            const int64_t o = base + (int64_t)j * WG + pos;
+            __builtin_nontemporal_store(x, &freq_out[o]);
+            typedef int i4v __attribute__((ext_vector_type(4)));
+            const i4v v = {id.origin_generation, id.origin_population, id.origin_threadID, id.reserved};
+            __builtin_nontemporal_store(v, reinterpret_cast<i4v *>(&ids_out[o]));
+        }
+        if (threadIdx.x == 0) {
+            unsigned int c = 0;
+            for (int k = 0; k < WG / 64; k++) c += sh_wcount[k];
+            B.cnt[gt] = c;
+        }
+    }
+    /* one atomic per wave, spread over 64 counters */
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
+    if (lane == 0 && live) atomicAdd(&B.live[(blockIdx.x * (WG / 64) + w) & 63], (unsigned long long)live);
+}
+
+/* ---- 3. per simulation: prefix of the block's survivor counts ---- */
+static __global__ __launch_bounds__(256) void k_bt_scan(BSim *__restrict__ sims, const BBufs B) {
+    const int s = blockIdx.x;
+    __shared__ int sh_w[4];
+    const int Kb = sims[s].Kb;
+    if (Kb == 0) return;   /* the whole workgroup */
+    const int T = sims[s].tiles, t0 = sims[s].t0, p0 = sims[s].p0;
+    const int out = sims[s].cur ^ 1;
+    const unsigned int *cnt = B.cnt + t0;
+    int *pre = (out ? B.pre[1] : B.pre[0]) + p0;
+    int *src0 = (out ? B.src0[1] : B.src0[0]) + t0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < T; base += 256 * 4) {
+        const int b = base + (int)threadIdx.x * 4;
+        int v[4], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = b + i < T ? (int)cnt[b + i] : 0;
+            sum += v[i];
+        }
+        int x = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) sh_w[w] = x;
+        __syncthreads();
+        int before = 0, round = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            before += k < w ? sh_w[k] : 0;
+            round += sh_w[k];
+        }
+        int run = carry + before + x - sum;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int idx = b + i;
+            if (idx < T) {
+                pre[idx] = run;
+                const int c = v[i];
+                for (int jn = (run + WG - 1) / WG; jn * WG <= run + c - 1; jn++) src0[jn] = idx;
+                run += c;
+            }
+        }
+        carry += round;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pre[T] = carry;
+        BSim &r = sims[s];
+        r.M = carry;
+        r.T = T;
+        r.cur = out;
+        r.g += Kb;
+        if (r.g >= r.G) r.status = BS_DONE;
+    }
+}
+
+/* ---- 4. status for the host: [active, stalled], live mutation-generations ---- */
+static __global__ __launch_bounds__(1024) void k_bt_status(const BSim *__restrict__ sims, const BBufs B,
+                                                           int *__restrict__ host) {
+    __shared__ int sh_a[16], sh_st[16];
+    int a = 0, st = 0;
+    for (int s = threadIdx.x; s < B.S; s += blockDim.x) {
+        const int v = sims[s].status;
+        a += v == BS_ACTIVE;
+        st += v == BS_STALLED;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        st += __shfl_xor(st, off);
+    }
+    if ((threadIdx.x & 63) == 0) { sh_a[threadIdx.x >> 6] = a; sh_st[threadIdx.x >> 6] = st; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, ts = 0;
+        for (int k = 0; k < (int)blockDim.x / 64; k++) { ta += sh_a[k]; ts += sh_st[k]; }
+        unsigned long long l = 0;
+        for (int k = 0; k < 64; k++) l += B.live[k];
+        host[0] = ta;
+        host[1] = ts;
+        *reinterpret_cast<unsigned long long *>(host + 2) = l;   /* pinned host words: read after the replay's event */
+    }
+}
+
+/* ---- re-layout: dense copy of the listed simulations' current states into new buffers ---- */
+struct BMove {
+    int64_t src_base;     /* first slot of the old region */
+    int64_t dst_base;     /* first slot of the new region */
+    const int *pre;       /* old fragmented layout (NULL: dense) */
+    const int *src0;
+    int T, M, cur, tile_first;   /* tile_first: the move's first tile in the copy grid */
+};
+
+static __global__ __launch_bounds__(WG) void k_bt_move(const BMove *__restrict__ mv, int nmv, int ntiles,
+                                                       float *const *__restrict__ freq_old,
+                                                       gf_mutid *const *__restrict__ ids_old,
+                                                       float *__restrict__ freq_new, gf_mutid *__restrict__ ids_new) {
+    __shared__ int sh_pre[WG + 1];
+    for (int gt = blockIdx.x; gt < ntiles; gt += gridDim.x) {
+        __syncthreads();
+        int lo = 0, hi = nmv - 1;   /* last move whose first tile <= gt */
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (mv[mid].tile_first <= gt) lo = mid; else hi = mid - 1;
+        }
+        const BMove m = mv[lo];
+        const int j = gt - m.tile_first;
+        const int li = j * WG + (int)threadIdx.x;
+        int s0 = 0;
+        if (m.pre && j * WG < m.M) s0 = load_frag_window(j, m.src0, m.pre, m.T, sh_pre);
+        if (li < m.M) {
+            const int64_t src = m.src_base + (m.pre ? frag_slot(li, s0, sh_pre, m.pre, m.T) : (int64_t)li);
+            freq_new[m.dst_base + li] = freq_old[m.cur][src];
+            ids_new[m.dst_base + li] = ids_old[m.cur][src];
+        }
+    }
+}
+
+/* ---- finalisation: SFS (and optionally a dense copy) of finished simulations ---- */
+struct BFin {
+    int64_t base;          /* first slot of the region */
+    const int *pre;        /* fragmented layout (NULL: dense) */
+    const unsigned int *cnt;
+    int T, M, cur, tile_first;
+    int64_t out_base;      /* first slot of the dense copy */
+};
+
+template <bool LDS>
+static __global__ __launch_bounds__(WG) void k_bt_final(const BFin *__restrict__ fin, int nfin, int ntiles,
+                                                        float *const *__restrict__ freq, gf_mutid *const *__restrict__ ids,
+                                                        int n, int half_n, const float *__restrict__ lc,
+                                                        unsigned long long *__restrict__ acc,
+                                                        float *__restrict__ copy_freq, gf_mutid *__restrict__ copy_ids) {
+    __shared__ unsigned long long sh[LDS ? SFS_LDS_BINS : 1];
+    for (int gt = blockIdx.x; gt < ntiles; gt += gridDim.x) {
+        int lo = 0, hi = nfin - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (fin[mid].tile_first <= gt) lo = mid; else hi = mid - 1;
+        }
+        const BFin f = fin[lo];
+        const int j = gt - f.tile_first;
+        unsigned long long *out = acc + (size_t)lo * (n + 2);
+        __syncthreads();
+        if (LDS) {
+            for (int k = threadIdx.x; k < n + 2; k += WG) sh[k] = 0;
+            __syncthreads();
+        }
+        /* slot of this lane and its compacted index */
+        bool has;
+        int ci;
+        if (f.pre) {
+            has = j < f.T && (unsigned int)threadIdx.x < f.cnt[j];
+            ci = has ? f.pre[j] + (int)threadIdx.x : 0;
+        } else {
+            ci = j * WG + (int)threadIdx.x;
+            has = ci < f.M;
+        }
+        if (has) {
+            const int64_t slot = f.base + (int64_t)j * WG + threadIdx.x;
+            const float x = freq[f.cur][slot];
+            sfs_add_mutation<LDS>(x, n, half_n, lc, sh, out);
+            if (copy_freq) {
+                copy_freq[f.out_base + ci] = x;
+                copy_ids[f.out_base + ci] = ids[f.cur][slot];
+            }
+        }
+        if (LDS) {
+            __syncthreads();
+            for (int k = threadIdx.x; k < n + 2; k += WG)
+                if (sh[k]) atomicAdd(&out[k], sh[k]);
+        }
+    }
+}
+
+/* ---------------- block-length autotuner ----------------
+ * Candidate block lengths are tried in turn, two rounds, each on one window
+ * of replays of a batch, and scored by the live mutation-generations per
+ * second of its windows; after the last window the best is kept.  The
+ * state is per batch shape (number of simulations, initial slots and
+ * longest run, each to a power of two, and the caller's cap), so short
+ * batches of one shape share one tuning and later batches use its result. */
+class BlockTuner {
+public:
+    static BlockTuner &get() {
+        static BlockTuner *t = new BlockTuner();
+        return *t;
+    }
+    static constexpr int kNumCand = 5;
+    static constexpr int kRounds = 2;
+    static int cand(int i) {
+        static const int c[kNumCand] = {8, 12, 16, 24, 32};
+        return c[i];
+    }
+    typedef std::tuple<int, int, int, int> Key;
+    static Key key(int S, int64_t slots, int maxG, int cap) {
+        auto lg = [](int64_t v) { int k = 0; while (v > 1) { v >>= 1; k++; } return k; };
+        return Key(lg(S), lg(slots), lg(maxG), cap);
+    }
+    /* block length of the next window; `measuring`: its time counts */
+    int next(const Key &k, int cap, bool &measuring) {
+        std::lock_guard<std::mutex> l(mu_);
+        State &s = st_[k];
+        while (!s.done && s.window < kRounds * kNumCand && cand(s.window % kNumCand) > cap) s.window++;
+        if (!s.done && s.window >= kRounds * kNumCand) finish(s, cap);
+        measuring = !s.done;
+        return s.done ? std::min(s.best, cap) : cand(s.window % kNumCand);
+    }
+    void record(const Key &k, int K, double work, double ms) {
+        std::lock_guard<std::mutex> l(mu_);
+        State &s = st_[k];
+        if (s.done) return;
+        s.work[K] += work;
+        s.ms[K] += ms;
+        s.window++;
+    }
+
+private:
+    struct State {
+        int window = 0;
+        bool done = false;
+        int best = 16;
+        std::map<int, double> work, ms;
+    };
+    void finish(State &s, int cap) {
+        double best = -1.0;
+        s.best = std::min(16, cap);
+        for (auto &kv : s.work) {
+            const double t = s.ms[kv.first];
+            const double rate = t > 0 ? kv.second / t : 0.0;
+            if (rate > best) { best = rate; s.best = kv.first; }
+        }
+        s.done = true;
+    }
+    std::mutex mu_;
+    std::map<Key, State> st_;
+};
 
 struct BatchRun {
     const gf_batch_sim *in;
@@ -89,68 +511,67 @@ struct BatchRun {
     bool keep;
     gf_batch_result *R;
     OwnedStream st;   /* before the buffers: they are released while it is alive */
-    std::vector<BatchSimDev> sims;
-    DevBuf<BatchSimDev> d_sims;
-    DevBuf<float> freqA, freqB;
-    DevBuf<gf_mutid> idsA, idsB;
-    int64_t cap = 0;
-    /* layout */
-    std::vector<int64_t> base, ncap;
-    std::vector<int> M;
-    std::vector<int> G;
-    KernelTimer t_adv, t_cmp;
+    std::vector<BSim> host;            /* host copy of the records (valid after sync_sims) */
+    DevBuf<BSim> d_sims;
+    DevBuf<float> freq[2];
+    DevBuf<gf_mutid> ids[2];
+    DevBuf<int> pre[2], src0[2], tile_sim, d_K;
+    DevBuf<unsigned int> cnt;
+    DevBuf<GenRow<1>> rows;
+    DevBuf<unsigned long long> live;
+    DevBuf<float *> d_fptr;
+    DevBuf<gf_mutid *> d_iptr;
+    PinBuf<int> h_status;              /* [active, stalled, live (2 words)] */
+    std::vector<int> layout;           /* simulations in the current layout, in tile order */
+    std::vector<char> finalised;
+    int64_t ntiles = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int relayouts = 0, blocks = 0;
 
-    int ensure_total(int64_t need) {
-        if (need <= cap && freqA.p) return GF_OK;
-        /* only called when the A buffers hold no live data */
-        int64_t c = std::max<int64_t>({need, cap + cap / 4, (int64_t)1 << 16});
-        c = (c + 255) / 256 * 256;
-        GF_TRY(freqA.alloc(c));
-        GF_TRY(freqB.alloc(c));
-        GF_TRY(idsA.alloc(c));
-        GF_TRY(idsB.alloc(c));
-        cap = c;
+    ~BatchRun() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    /* tiles for a simulation holding M slots that may gain K generations of
+     * new mutations (7 standard deviations over their mean), with a quarter
+     * of headroom for the survivors */
+    static int tiles_for(int64_t M, double lam, int K) {
+        const double need = 1.25 * (double)M + lam * K + 8.0 * std::sqrt(lam * K + 1.0) + 256.0;
+        return (int)std::min<double>(std::ceil(need / WG), 2e9 / WG);
+    }
+
+    int sync_sims() {
+        GF_HIP(hipMemcpyAsync(host.data(), d_sims.p, (size_t)S * sizeof(BSim), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
         return GF_OK;
     }
 
-    int ensure_B(int64_t need) {
-        if (need <= (int64_t)freqB.n && freqB.p) return GF_OK;
-        int64_t c = std::max<int64_t>(need, (int64_t)freqB.n + (int64_t)freqB.n / 4);
-        c = (c + 255) / 256 * 256;
-        freqB.release();
-        idsB.release();
-        GF_TRY(freqB.alloc(c));
-        GF_TRY(idsB.alloc(c));
-        return GF_OK;
-    }
-
-    int init() {
-        sims.resize(S);
+    /* ----- mutation-selection equilibrium of every simulation into buffer 0, dense ----- */
+    int init(int Kmax) {
+        host.assign(S, BSim{});
         for (int s = 0; s < S; s++) {
             const gf_batch_sim &b = in[s];
-            BatchSimDev &d = sims[s];
-            memset(&d, 0, sizeof d);
+            BSim &d = host[s];
             const int N1 = b.N1 > 0 ? b.N1 : b.N;
             d.N = N1; d.F = b.F; d.h = b.h; d.mu = b.mu; d.s = b.s; d.L = b.num_sites;
             d.G = b.num_generations;
             d.k0 = (uint32_t)b.seed1; d.k1 = (uint32_t)b.seed2;
             d.Nc = N1 > 0 ? nchrom(N1, b.F) : 0;
             d.fN = (float)d.Nc;
+            d.invN = d.Nc > 0 ? 1.0 / (double)d.fN : 0.0;
             d.c1 = (b.F + b.h) - (b.h * b.F);
             d.c2 = (b.F + (2.0f * b.h)) - ((2.0f * b.h) * b.F);
             d.newfreq = d.Nc > 0 ? 1.0f / (float)d.Nc : 0.0f;
             d.neutral = b.s == 0.0f;
+            d.T = -1;
+            d.status = d.G > 0 ? BS_ACTIVE : BS_DONE;
             if (b.N < 0 || N1 < 0 || b.F < 0 || b.F > 1 || b.mu < 0 || b.num_generations < 0) {
                 set_error("gf_batch_run: simulation " + std::to_string(s) + " has invalid parameters");
                 return GF_ERR_PARAM;
             }
         }
-        GF_TRY(d_sims.alloc(S));
-        GF_HIP(hipMemcpyAsync(d_sims.p, sims.data(), S * sizeof(BatchSimDev), hipMemcpyHostToDevice, st));
-        G.resize(S);
-        for (int s = 0; s < S; s++) G[s] = in[s].num_generations;
-
-        /* ----- mutation-selection equilibrium of every simulation ----- */
         std::vector<MseSeg> segs;
         std::vector<int> seg_sim;
         int64_t coff = 0, aoff = 0, choff = 0;
@@ -161,81 +582,307 @@ struct BatchRun {
             MseSeg m{};
             m.Nc = Nc0; m.Nind = in[s].N; m.F = in[s].F; m.h = in[s].h; m.mu = in[s].mu;
             m.sa = 0.0f; m.sb = in[s].s; m.L = in[s].num_sites;
-            m.k0 = sims[s].k0; m.k1 = sims[s].k1; m.pop = 0;
-            m.neutral = sims[s].neutral; m.preserve = 0;
+            m.k0 = host[s].k0; m.k1 = host[s].k1; m.pop = 0;
+            m.neutral = host[s].neutral; m.preserve = 0;
             m.class_off = coff; m.area_off = aoff; m.chunk_off = choff;
             coff += m.Nc - 1; aoff += m.Nc; choff += (m.Nc + MSE_CHUNK - 1) / MSE_CHUNK;
             if (!m.neutral) any_sel = true;
             segs.push_back(m);
             seg_sim.push_back(s);
         }
-        base.assign(S, 0);
-        ncap.assign(S, 0);
-        M.assign(S, 0);
+        std::vector<int64_t> Minit(S, 0);
         const int ns = (int)segs.size();
-        if (ns == 0) return ensure_total(256);
-        std::vector<int64_t> offs(3 * (size_t)ns);
-        for (int k = 0; k < ns; k++) { offs[k] = segs[k].class_off; offs[ns + k] = segs[k].area_off; offs[2 * ns + k] = segs[k].chunk_off; }
+        std::vector<int64_t> scan;
         DevBuf<MseSeg> d_segs;
-        DevBuf<int64_t> d_offs;
-        GF_TRY(d_segs.alloc(ns));
-        GF_TRY(d_offs.alloc(3 * (size_t)ns));
-        GF_HIP(hipMemcpyAsync(d_segs.p, segs.data(), ns * sizeof(MseSeg), hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(d_offs.p, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        DevBuf<int64_t> d_offs, d_scan, d_scratch, d_total;
         const unsigned grid = (unsigned)std::max(1, num_cus() * 8);
-        DevBuf<float> d_area, d_tot;
-        if (any_sel) {
-            GF_TRY(d_area.alloc((size_t)aoff));
-            GF_TRY(d_tot.alloc((size_t)choff));
-            hipLaunchKernelGGL(k_mse_area, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + ns, ns, aoff, d_area.p);
-            hipLaunchKernelGGL(k_mse_chunk_local, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + 2 * ns, ns, choff, d_area.p, d_tot.p);
-            hipLaunchKernelGGL(k_mse_chunk_prefix, dim3((ns + 63) / 64), dim3(64), 0, st, d_segs.p, ns, d_tot.p);
-            hipLaunchKernelGGL(k_mse_chunk_add, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + ns, ns, aoff, d_area.p, d_tot.p);
+        if (ns > 0) {
+            std::vector<int64_t> offs(3 * (size_t)ns);
+            for (int k = 0; k < ns; k++) { offs[k] = segs[k].class_off; offs[ns + k] = segs[k].area_off; offs[2 * ns + k] = segs[k].chunk_off; }
+            GF_TRY(d_segs.alloc(ns));
+            GF_TRY(d_offs.alloc(3 * (size_t)ns));
+            GF_HIP(hipMemcpyAsync(d_segs.p, segs.data(), ns * sizeof(MseSeg), hipMemcpyHostToDevice, st));
+            GF_HIP(hipMemcpyAsync(d_offs.p, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            DevBuf<float> d_area, d_tot;
+            if (any_sel) {
+                GF_TRY(d_area.alloc((size_t)aoff));
+                GF_TRY(d_tot.alloc((size_t)choff));
+                hipLaunchKernelGGL(k_mse_area, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + ns, ns, aoff, d_area.p);
+                hipLaunchKernelGGL(k_mse_chunk_local, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + 2 * ns, ns, choff, d_area.p, d_tot.p);
+                hipLaunchKernelGGL(k_mse_chunk_prefix, dim3((ns + 63) / 64), dim3(64), 0, st, d_segs.p, ns, d_tot.p);
+                hipLaunchKernelGGL(k_mse_chunk_add, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p + ns, ns, aoff, d_area.p, d_tot.p);
+                GF_HIP(hipGetLastError());
+            }
+            DevBuf<int> d_counts;
+            GF_TRY(d_counts.alloc((size_t)coff));
+            GF_TRY(d_scan.alloc((size_t)coff + 1));
+            GF_TRY(d_total.alloc(1));
+            hipLaunchKernelGGL(k_mse_counts, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p, ns, coff,
+                               any_sel ? d_area.p : nullptr, d_counts.p);
+            GF_HIP(hipGetLastError());
+            GF_TRY(scan_counts(d_counts.p, coff, d_scan.p, d_total.p, st, d_scratch));
+            GF_HIP(hipMemcpyAsync(d_scan.p + coff, d_total.p, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+            scan.resize((size_t)coff + 1);
+            GF_HIP(hipMemcpyAsync(scan.data(), d_scan.p, scan.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+            GF_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < ns; k++) {
+                const int64_t tot = scan[segs[k].class_off + segs[k].Nc - 1] - scan[segs[k].class_off];
+                if (tot > 2000000000LL) { set_error("gf_batch_run: initial mutation count exceeds 2e9"); return GF_ERR_UNSUPPORTED; }
+                Minit[seg_sim[k]] = tot;
+            }
+        }
+        /* the layout: every simulation with generations to run, its tiles
+         * sized for its equilibrium count and a block of new mutations */
+        layout.clear();
+        int64_t t = 0;
+        int64_t pcount = 0;
+        for (int s = 0; s < S; s++) {
+            BSim &d = host[s];
+            d.M = (int)Minit[s];
+            d.t0 = (int)t;
+            d.p0 = (int)(t + pcount);
+            const double lam = (double)d.mu * (double)d.Nc * (double)d.L;
+            /* a population that grows after generation 0 keeps more of its
+             * new mutations for a while: room for ~4 ln(G) generations more */
+            const int grow = d.N > in[s].N ? (int)std::ceil(4.0 * std::log1p((double)d.G)) : 0;
+            d.nt = std::max(tiles_for(d.M, lam, Kmax + grow), (int)((d.M + WG - 1) / WG));
+            t += d.nt;
+            pcount++;
+            layout.push_back(s);
+            if (t > 2000000000LL / WG) { set_error("gf_batch_run: more than 2e9 slots"); return GF_ERR_UNSUPPORTED; }
+        }
+        GF_TRY(allocate_layout(t));
+        if (ns > 0) {
+            std::vector<int64_t> out_offs(ns);
+            for (int k = 0; k < ns; k++) {
+                const int s = seg_sim[k];
+                segs[k].slot_base = (int64_t)host[s].t0 * WG;
+                segs[k].sim_base = segs[k].slot_base;
+                out_offs[k] = scan[segs[k].class_off];
+            }
+            GF_HIP(hipMemcpyAsync(d_segs.p, segs.data(), ns * sizeof(MseSeg), hipMemcpyHostToDevice, st));
+            GF_HIP(hipMemcpyAsync(d_offs.p, out_offs.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            const int64_t total = scan[coff];
+            if (total > 0)
+                hipLaunchKernelGGL(k_mse_fill<1>, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p, ns, total, d_scan.p,
+                                   freq[0].p, (int64_t)freq[0].n, ids[0].p);
             GF_HIP(hipGetLastError());
         }
-        DevBuf<int> d_counts;
-        DevBuf<int64_t> d_scan, d_scratch, d_total;
-        GF_TRY(d_counts.alloc((size_t)coff));
-        GF_TRY(d_scan.alloc((size_t)coff + 1));
-        GF_TRY(d_total.alloc(1));
-        hipLaunchKernelGGL(k_mse_counts, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p, ns, coff,
-                           any_sel ? d_area.p : nullptr, d_counts.p);
-        GF_HIP(hipGetLastError());
-        GF_TRY(scan_counts(d_counts.p, coff, d_scan.p, d_total.p, st, d_scratch));
-        GF_HIP(hipMemcpyAsync(d_scan.p + coff, d_total.p, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
-        /* per-segment first offsets: gather with a copy per segment would be
-         * S round trips; copy the whole scan instead */
-        std::vector<int64_t> scan((size_t)coff + 1);
-        GF_HIP(hipMemcpyAsync(scan.data(), d_scan.p, scan.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        GF_HIP(hipStreamSynchronize(st));
-        std::vector<int64_t> out_offs(ns);
-        int64_t b = 0;
-        for (int k = 0; k < ns; k++) {
-            const int s = seg_sim[k];
-            const int64_t first = scan[segs[k].class_off];
-            const int64_t last = scan[segs[k].class_off + segs[k].Nc - 1];
-            const int64_t tot = last - first;
-            if (tot > 2000000000LL) { set_error("gf_batch_run: initial mutation count exceeds 2e9"); return GF_ERR_UNSUPPORTED; }
-            M[s] = (int)tot;
-            base[s] = b;
-            ncap[s] = (tot + 255) / 256 * 256;
-            segs[k].slot_base = b;
-            segs[k].sim_base = b;
-            out_offs[k] = first;   /* position in the global scan */
-            b += ncap[s];
-        }
-        GF_TRY(ensure_total(b + 256));
-        /* the fill kernel maps output index j (global scan space) to segment
-         * k by out_offs and writes slot slot_base + (j - out_offs[k]) */
-        GF_HIP(hipMemcpyAsync(d_segs.p, segs.data(), ns * sizeof(MseSeg), hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(d_offs.p, out_offs.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        const int64_t total = scan[coff];
-        if (total > 0)
-            hipLaunchKernelGGL(k_mse_fill<1>, dim3(grid), dim3(256), 0, st, d_segs.p, d_offs.p, ns, total, d_scan.p,
-                               freqA.p, (int64_t)cap, idsA.p);
-        GF_HIP(hipGetLastError());
+        GF_HIP(hipMemcpyAsync(d_sims.p, host.data(), (size_t)S * sizeof(BSim), hipMemcpyHostToDevice, st));
         GF_HIP(hipStreamSynchronize(st));
         return GF_OK;
+    }
+
+    /* buffers and tables of a layout of `t` tiles (host[] holds t0/nt/p0) */
+    int allocate_layout(int64_t t) {
+        ntiles = t;
+        const size_t slots = (size_t)std::max<int64_t>(t, 1) * WG;
+        for (int b = 0; b < 2; b++) {
+            GF_TRY(freq[b].alloc(slots));
+            GF_TRY(ids[b].alloc(slots));
+            GF_TRY(pre[b].alloc((size_t)t + layout.size() + 1));
+            GF_TRY(src0[b].alloc((size_t)std::max<int64_t>(t, 1)));
+        }
+        GF_TRY(cnt.alloc((size_t)std::max<int64_t>(t, 1)));
+        std::vector<int> ts((size_t)std::max<int64_t>(t, 1), 0);
+        for (int s : layout)
+            for (int j = 0; j < host[s].nt; j++) ts[(size_t)host[s].t0 + j] = s;
+        GF_TRY(tile_sim.alloc(ts.size()));
+        GF_HIP(hipMemcpyAsync(tile_sim.p, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        float *fp[2] = {freq[0].p, freq[1].p};
+        gf_mutid *ip[2] = {ids[0].p, ids[1].p};
+        GF_TRY(d_fptr.alloc(2));
+        GF_TRY(d_iptr.alloc(2));
+        GF_HIP(hipMemcpyAsync(d_fptr.p, fp, sizeof fp, hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_iptr.p, ip, sizeof ip, hipMemcpyHostToDevice, st));
+        GF_HIP(hipStreamSynchronize(st));   /* the host arrays above are locals */
+        return GF_OK;
+    }
+
+    BBufs bufs() const {
+        BBufs B;
+        for (int b = 0; b < 2; b++) {
+            B.freq[b] = freq[b].p; B.ids[b] = ids[b].p; B.pre[b] = pre[b].p; B.src0[b] = src0[b].p;
+        }
+        B.cnt = cnt.p;
+        B.rows = rows.p;
+        B.tile_sim = tile_sim.p;
+        B.live = live.p;
+        B.K = d_K.p;
+        B.ntiles = (int)ntiles;
+        B.S = S;
+        return B;
+    }
+
+    /* one block = rows, advance, scan, status: captured once per layout */
+    int capture() {
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        const BBufs B = bufs();
+        const int grid = std::max(1, std::min<int>((int)ntiles, num_cus() * 8));
+        GF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(k_bt_rows, dim3((S + 127) / 128), dim3(128), 0, st, d_sims.p, B);
+        hipLaunchKernelGGL(k_bt_advance, dim3(grid), dim3(WG), 0, st, (const BSim *)d_sims.p, B);
+        hipLaunchKernelGGL(k_bt_scan, dim3(S), dim3(256), 0, st, d_sims.p, B);
+        hipLaunchKernelGGL(k_bt_status, dim3(1), dim3(1024), 0, st, (const BSim *)d_sims.p, B, h_status.p);
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess) {
+            set_error(std::string("gf_batch_run: graph capture failed: ") + hipGetErrorString(e));
+            return GF_ERR_DEVICE;
+        }
+        GF_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return GF_OK;
+    }
+
+    /* finalise the listed simulations (all finished) from the current buffers */
+    int finalise(const std::vector<int> &done) {
+        const int nd = (int)done.size();
+        if (nd == 0) return GF_OK;
+        std::vector<BFin> fin(nd);
+        int64_t tiles = 0, total = 0;
+        for (int k = 0; k < nd; k++) {
+            const BSim &b = host[done[k]];
+            BFin &f = fin[k];
+            f.base = (int64_t)b.t0 * WG;
+            f.pre = b.T >= 0 ? pre[b.cur].p + b.p0 : nullptr;
+            f.cnt = cnt.p + b.t0;
+            f.T = b.T;
+            f.M = b.M;
+            f.cur = b.cur;
+            f.tile_first = (int)tiles;
+            f.out_base = total;
+            tiles += b.T >= 0 ? b.T : (b.M + WG - 1) / WG;
+            total += b.M;
+        }
+        std::vector<float> lc;
+        int half_n;
+        log_binom_table(n, lc, half_n);
+        DevBuf<BFin> d_fin;
+        DevBuf<float> d_lc;
+        DevBuf<unsigned long long> d_acc;
+        DevBuf<float> cf;
+        DevBuf<gf_mutid> ci;
+        GF_TRY(d_fin.alloc(nd));
+        GF_TRY(d_lc.alloc(lc.size()));
+        GF_TRY(d_acc.alloc((size_t)nd * (n + 2)));
+        if (keep && total > 0) {
+            GF_TRY(cf.alloc((size_t)total));
+            GF_TRY(ci.alloc((size_t)total));
+        }
+        GF_HIP(hipMemcpyAsync(d_fin.p, fin.data(), nd * sizeof(BFin), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(d_lc.p, lc.data(), lc.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemsetAsync(d_acc.p, 0, (size_t)nd * (n + 2) * sizeof(unsigned long long), st));
+        if (tiles > 0) {
+            const int grid = (int)std::min<int64_t>(tiles, (int64_t)num_cus() * 8);
+            float *cfp = keep && total > 0 ? cf.p : nullptr;
+            gf_mutid *cip = keep && total > 0 ? ci.p : nullptr;
+            if (n + 2 <= SFS_LDS_BINS)
+                hipLaunchKernelGGL((k_bt_final<true>), dim3(grid), dim3(WG), 0, st, d_fin.p, nd, (int)tiles, d_fptr.p,
+                                   d_iptr.p, n, half_n, d_lc.p, d_acc.p, cfp, cip);
+            else
+                hipLaunchKernelGGL((k_bt_final<false>), dim3(grid), dim3(WG), 0, st, d_fin.p, nd, (int)tiles, d_fptr.p,
+                                   d_iptr.p, n, half_n, d_lc.p, d_acc.p, cfp, cip);
+            GF_HIP(hipGetLastError());
+        }
+        std::vector<unsigned long long> acc((size_t)nd * (n + 2));
+        GF_HIP(hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        std::vector<float> hf;
+        std::vector<gf_mutid> hi;
+        if (keep && total > 0) {
+            hf.resize((size_t)total);
+            hi.resize((size_t)total);
+            GF_TRY(readback(hf.data(), cf.p, (size_t)total * sizeof(float), st));
+            GF_TRY(readback(hi.data(), ci.p, (size_t)total * sizeof(gf_mutid), st));
+        }
+        GF_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < nd; k++) {
+            const int s = done[k];
+            const BSim &b = host[s];
+            R->num_mutations[s] = b.M;
+            finish_sfs(&acc[(size_t)k * (n + 2)], acc[(size_t)k * (n + 2) + n + 1], n, b.M, in[s].num_sites,
+                       &R->sfs[(size_t)s * n]);
+            if (keep) {
+                R->freq[s].assign(hf.begin() + fin[k].out_base, hf.begin() + fin[k].out_base + b.M);
+                R->ids[s].assign(hi.begin() + fin[k].out_base, hi.begin() + fin[k].out_base + b.M);
+            }
+            finalised[s] = 1;
+        }
+        return GF_OK;
+    }
+
+    /* finalise finished simulations, then move the others (dense) into a
+     * new layout: stalled ones with at least twice their tiles */
+    int relayout(int K) {
+        GF_TRY(sync_sims());
+        std::vector<int> done, keep_s;
+        for (int s : layout) {
+            if (finalised[s]) continue;
+            if (host[s].status == BS_DONE) done.push_back(s);
+            else keep_s.push_back(s);
+        }
+        GF_TRY(finalise(done));
+        /* the new regions */
+        std::vector<BSim> old(host);
+        int64_t t = 0;
+        int k = 0;
+        std::vector<BMove> mv;
+        int64_t tiles = 0;
+        for (int s : keep_s) {
+            BSim &b = host[s];
+            const double lam = (double)b.mu * (double)b.Nc * (double)b.L;
+            int nt = b.nt;
+            if (b.status == BS_STALLED) {
+                nt = std::max(2 * b.nt, tiles_for(b.M, lam, K));
+                b.status = BS_ACTIVE;
+            }
+            b.nt = nt;
+            b.t0 = (int)t;
+            b.p0 = (int)(t + k);
+            BMove m;
+            m.src_base = (int64_t)old[s].t0 * WG;
+            m.dst_base = (int64_t)b.t0 * WG;
+            m.pre = old[s].T >= 0 ? pre[old[s].cur].p + old[s].p0 : nullptr;
+            m.src0 = old[s].T >= 0 ? src0[old[s].cur].p + old[s].t0 : nullptr;
+            m.T = old[s].T;
+            m.M = old[s].M;
+            m.cur = old[s].cur;
+            m.tile_first = (int)tiles;
+            tiles += (b.M + WG - 1) / WG;
+            mv.push_back(m);
+            b.T = -1;
+            b.cur = 0;
+            t += nt;
+            k++;
+            if (t > 2000000000LL / WG) { set_error("gf_batch_run: more than 2e9 slots"); return GF_ERR_UNSUPPORTED; }
+        }
+        /* copy into fresh buffers, then swap them in */
+        DevBuf<float> nf;
+        DevBuf<gf_mutid> ni;
+        const size_t slots = (size_t)std::max<int64_t>(t, 1) * WG;
+        GF_TRY(nf.alloc(slots));
+        GF_TRY(ni.alloc(slots));
+        if (tiles > 0) {
+            DevBuf<BMove> d_mv;
+            GF_TRY(d_mv.alloc(mv.size()));
+            GF_HIP(hipMemcpyAsync(d_mv.p, mv.data(), mv.size() * sizeof(BMove), hipMemcpyHostToDevice, st));
+            const int grid = (int)std::min<int64_t>(tiles, (int64_t)num_cus() * 8);
+            hipLaunchKernelGGL(k_bt_move, dim3(grid), dim3(WG), 0, st, d_mv.p, (int)mv.size(), (int)tiles,
+                               d_fptr.p, d_iptr.p, nf.p, ni.p);
+            GF_HIP(hipGetLastError());
+            GF_HIP(hipStreamSynchronize(st));   /* before the old buffers and tables go */
+        }
+        freq[0].swap(nf);
+        ids[0].swap(ni);
+        nf.release();
+        ni.release();
+        freq[1].release();
+        ids[1].release();
+        for (int b = 0; b < 2; b++) { pre[b].release(); src0[b].release(); }
+        layout = keep_s;
+        GF_TRY(allocate_layout(t));
+        GF_HIP(hipMemcpyAsync(d_sims.p, host.data(), (size_t)S * sizeof(BSim), hipMemcpyHostToDevice, st));
+        relayouts++;
+        return capture();
     }
 
     int run() {
@@ -244,268 +891,104 @@ struct BatchRun {
         R->sfs.assign((size_t)S * n, 0.0f);
         R->num_mutations.assign(S, 0);
         if (keep) { R->freq.resize(S); R->ids.resize(S); }
-        t_adv.stride = t_cmp.stride = timing_stride();
+        finalised.assign(S, 0);
         auto t0 = std::chrono::steady_clock::now();
         GF_HIP(st.create());
         StreamScope scope(st);
-        hipEvent_t i0, i1;
+        hipEvent_t i0, i1, w0, w1;
         GF_HIP(hipEventCreate(&i0));
         GF_HIP(hipEventCreate(&i1));
+        GF_HIP(hipEventCreate(&w0));
+        GF_HIP(hipEventCreate(&w1));
+        struct Ev { hipEvent_t *e[4]; ~Ev() { for (auto p : e) if (*p) (void)hipEventDestroy(*p); } } ev_guard{{&i0, &i1, &w0, &w1}};
         GF_HIP(hipEventRecord(i0, st));
-        GF_TRY(init());
+
+        /* block length: the caller's cap (compaction interval, launch
+         * length) or the tuner */
+        int cap = std::min(BK_MAX, interval);
+        const int forced = generations_per_launch(0);
+        if (forced > 0) cap = std::min(cap, forced);
+        GF_TRY(d_sims.alloc(S));
+        GF_TRY(init(cap));
         GF_HIP(hipEventRecord(i1, st));
-
-        const int kb = std::max(1, std::min({KMAX, interval > 0 ? interval : KMAX, generations_per_launch(16)}));
-        DevBuf<unsigned long long> d_live;   /* 64 partial counters, summed at the end */
-        GF_TRY(d_live.alloc(64));
-        DevBuf<unsigned int> d_ticket;
-        GF_TRY(d_ticket.alloc(1));
-        GF_HIP(hipMemsetAsync(d_live.p, 0, 64 * sizeof(unsigned long long), st));
-
-        /* device scratch */
-        DevBuf<int> d_tile_seg, d_meta, d_active, d_tot;
-        DevBuf<int64_t> d_base64;
-        DevBuf<unsigned long long> d_masks, d_acc;
-        DevBuf<unsigned int> d_tcount, d_toff;
-        DevBuf<GenRow<1>> d_rows;
-        DevBuf<SegLaunch<1>> d_sl;
-        DevBuf<float> d_lc;
-        PinBuf<int> h_i;
-        PinBuf<int64_t> h_64;
-        PinBuf<SegLaunch<1>> h_sl;
-
-        std::vector<int> layout;   /* simulations currently in the A buffers */
-        for (int s = 0; s < S; s++) if (G[s] >= 0) layout.push_back(s);
-        std::vector<int> Mend(M);
-        bool flags_ready = false;   /* d_masks / d_tcount hold the current layout's flags */
-        int g = 0;
-        std::vector<char> finished(S, 0);
-        for (;;) {
-            /* simulations that still have generations to run */
-            std::vector<int> active;
-            for (int s : layout) if (G[s] > g) active.push_back(s);
-            const int nl = (int)layout.size();
-            const int na = (int)active.size();
-            const int K = kb;
-            /* ---- 1. segmented flag over the current layout (the previous
-             * block's advance already wrote the flags; only the first block
-             * runs the flag pass) ---- */
-            std::vector<int> tile_first(nl), tile_num(nl);
-            int ntiles = 0;
-            for (int k = 0; k < nl; k++) {
-                const int s = layout[k];
-                tile_first[k] = ntiles;
-                const int nt = (Mend[s] + WG - 1) / WG;
-                tile_num[k] = nt;
-                ntiles += nt;
-            }
-            /* pinned host staging: [tile_first | tile_num | seg_M | present | preserve | total | active | tot] */
-            const size_t nmeta = 6 * (size_t)nl + 2 * (size_t)na + 8;
-            GF_TRY(h_i.alloc(nmeta));
-            GF_TRY(d_meta.alloc(nmeta));
-            int *hm = h_i.p;
-            int *h_tile_first = hm, *h_tile_num = h_tile_first + nl, *h_M = h_tile_num + nl,
-                *h_present = h_M + nl, *h_pres = h_present + nl, *h_total = h_pres + nl, *h_active = h_total + nl,
-                *h_tot = h_active + na;
-            for (int k = 0; k < nl; k++) {
-                const int s = layout[k];
-                h_tile_first[k] = tile_first[k];
-                h_tile_num[k] = tile_num[k];
-                h_M[k] = Mend[s];
-                h_present[k] = sims[s].N > 0 ? 1 : 0;
-                h_pres[k] = 0;
-                h_total[k] = 0;
-            }
-            for (int a = 0; a < na; a++) { h_active[a] = active[a]; h_tot[a] = 0; }
-            const ptrdiff_t o_tile_first = 0, o_tile_num = o_tile_first + nl, o_M = o_tile_num + nl,
-                            o_present = o_M + nl, o_pres = o_present + nl, o_total = o_pres + nl,
-                            o_active = o_total + nl, o_tot = o_active + na;
-            GF_TRY(h_64.alloc(2 * (size_t)nl + 2));
-            GF_TRY(d_base64.alloc(2 * (size_t)nl + 2));
-            for (int k = 0; k < nl; k++) h_64.p[k] = base[layout[k]];
-            GF_HIP(hipMemcpyAsync(d_meta.p, hm, nmeta * sizeof(int), hipMemcpyHostToDevice, st));
-            GF_HIP(hipMemcpyAsync(d_base64.p, h_64.p, (size_t)nl * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            /* the advance sized d_masks / d_tcount for exactly these tiles
-             * and filled them: no reallocation in between */
-            if (!flags_ready) {
-                GF_TRY(d_masks.alloc((size_t)std::max(ntiles, 1) * (WG / 64)));
-                GF_TRY(d_tcount.alloc((size_t)std::max(ntiles, 1)));
-            }
-            GF_TRY(d_toff.alloc((size_t)std::max(ntiles, 1)));
-            hipEvent_t ce = nullptr;
-            t_cmp.begin(st, ce);
-            if (ntiles > 0) {
-                if (!flags_ready)
-                    hipLaunchKernelGGL(k_flag<1>, dim3(ntiles), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
-                                       d_base64.p, d_meta.p + o_M, d_meta.p + o_present, d_meta.p + o_tile_first, nl,
-                                       d_masks.p, d_tcount.p);
-                hipLaunchKernelGGL(k_scan_tiles, dim3(nl), dim3(1024), 0, st, d_tcount.p, d_meta.p + o_tile_first,
-                                   d_meta.p + o_tile_num, d_toff.p, d_meta.p + o_total);
-                GF_HIP(hipGetLastError());
-            }
-            /* ---- 1b. next block's rows and new-mutation totals ---- */
-            if (na > 0) {
-                GF_TRY(d_rows.alloc((size_t)na * K));
-                hipLaunchKernelGGL(k_batch_rows, dim3((na + 127) / 128), dim3(128), 0, st, d_sims.p,
-                                   d_meta.p + o_active, na, g, K, d_rows.p, d_meta.p + o_tot);
-                GF_HIP(hipGetLastError());
-            }
-            /* ---- 2. read back survivors and new-mutation totals ---- */
-            /* [total (nl) | active (na) | tot (na)] */
-            GF_HIP(hipMemcpyAsync(h_total, d_meta.p + o_total, ((size_t)nl + 2 * (size_t)na) * sizeof(int), hipMemcpyDeviceToHost, st));
-            GF_HIP(hipStreamSynchronize(st));
-            /* ---- 3. new layout: survivors + the block's new mutations ---- */
-            std::vector<int64_t> nbase(nl);
-            std::vector<int> tot_of(S, 0);
-            for (int a = 0; a < na; a++) tot_of[active[a]] = h_tot[a];
-            int64_t b = 0;
-            for (int k = 0; k < nl; k++) {
-                const int s = layout[k];
-                const int64_t need = (int64_t)h_total[k] + tot_of[s];
-                if (need > 2000000000LL) { set_error("gf_batch_run: simulation exceeds 2e9 slots"); return GF_ERR_UNSUPPORTED; }
-                nbase[k] = b;
-                b += (need + 255) / 256 * 256;
-            }
-            GF_TRY(ensure_B(b + 256));
-            for (int k = 0; k < nl; k++) h_64.p[nl + k] = nbase[k];
-            GF_HIP(hipMemcpyAsync(d_base64.p + nl, h_64.p + nl, (size_t)nl * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            if (ntiles > 0) {
-                hipLaunchKernelGGL(k_scatter<1>, dim3((ntiles + SCATTER_TILES - 1) / SCATTER_TILES), dim3(WG), 0, st,
-                                   freqA.p, (int64_t)cap, idsA.p, freqB.p, (int64_t)freqB.n, idsB.p, d_base64.p,
-                                   d_base64.p + nl, d_meta.p + o_pres, d_meta.p + o_tile_first, nl, d_masks.p, d_toff.p,
-                                   ntiles);
-                GF_HIP(hipGetLastError());
-            }
-            t_cmp.end(ce);
-            freqA.swap(freqB);
-            idsA.swap(idsB);
-            cap = (int64_t)freqA.n;
-            for (int k = 0; k < nl; k++) {
-                const int s = layout[k];
-                base[s] = nbase[k];
-                M[s] = h_total[k];
-            }
-            /* ---- 4. finalise simulations whose last generation is g ---- */
-            std::vector<int> done;
-            for (int s : layout) if (G[s] <= g && !finished[s]) done.push_back(s);
-            if (!done.empty()) GF_TRY(finalise(done, d_acc, d_lc));
-            for (int s : done) finished[s] = 1;
-            if (na == 0) break;
-            /* ---- 5. one advance launch over every live simulation ---- */
-            GF_TRY(h_sl.alloc(na));
-            GF_TRY(d_sl.alloc(na));
-            std::vector<int> atile_first(na);
-            int ntl = 0;
-            for (int a = 0; a < na; a++) {
-                const int s = active[a];
-                SegLaunch<1> &L = h_sl.p[a];
-                memset(&L, 0, sizeof L);
-                L.rows = d_rows.p + (size_t)a * K;
-                L.live = d_live.p;
-                L.base = base[s];
-                L.M0 = M[s];
-                L.Mend = M[s] + tot_of[s];
-                L.cum_base = 0;
-                L.K = std::min(K, G[s] - g);
-                L.g0 = g;
-                L.t0 = 0;
-                L.k0 = sims[s].k0;
-                L.k1 = sims[s].k1;
-                atile_first[a] = ntl;
-                ntl += (L.Mend + WG - 1) / WG;
-                Mend[s] = L.Mend;
-            }
-            GF_TRY(d_tile_seg.alloc((size_t)na + 1));
-            GF_HIP(hipMemcpyAsync(d_sl.p, h_sl.p, na * sizeof(SegLaunch<1>), hipMemcpyHostToDevice, st));
-            GF_HIP(hipMemcpyAsync(d_tile_seg.p, atile_first.data(), (size_t)na * sizeof(int), hipMemcpyHostToDevice, st));
-            /* the advance writes the next block's compaction flags */
-            GF_TRY(d_masks.alloc((size_t)std::max(ntl, 1) * (WG / 64)));
-            GF_TRY(d_tcount.alloc((size_t)std::max(ntl, 1)));
-            flags_ready = true;
-            if (ntl > 0) {
-                const int grid = std::min(ntl, num_cus() * 8);
-                GF_HIP(hipMemsetAsync(d_ticket.p, 0, sizeof(unsigned int), st));
-                hipEvent_t ae = nullptr;
-                t_adv.begin(st, ae);
-                hipLaunchKernelGGL((k_advance<1, true>), dim3(grid), dim3(WG), 0, st, freqA.p, (int64_t)cap, idsA.p,
-                                   h_sl.p[0], (const SegLaunch<1> *)d_sl.p, (const int *)d_tile_seg.p, na, ntl,
-                                   d_ticket.p, d_masks.p, d_tcount.p);
-                GF_HIP(hipGetLastError());
-                t_adv.end(ae);
-            }
-            /* the host copy of tmeta must outlive the async copy */
-            GF_HIP(hipStreamSynchronize(st));
-            /* finished simulations leave the layout; a simulation whose last
-             * generation falls inside the block ran L.K < K generations and
-             * is finalised at the top of the next iteration (G[s] <= g) */
-            layout = active;
-            g += K;
-        }
+        int maxG = 0;
+        for (int s = 0; s < S; s++) maxG = std::max(maxG, host[s].G);
+        GF_TRY(rows.alloc((size_t)S * BK_MAX));
+        GF_TRY(live.alloc(64));
+        GF_TRY(d_K.alloc(1));
+        GF_TRY(h_status.alloc(4));
+        GF_HIP(hipMemsetAsync(live.p, 0, 64 * sizeof(unsigned long long), st));
+        const BlockTuner::Key key = BlockTuner::key(S, ntiles * WG, maxG, cap);
+        bool measuring = false;
+        int K = forced > 0 ? cap : BlockTuner::get().next(key, cap, measuring);
+        GF_HIP(hipMemcpyAsync(d_K.p, &K, sizeof(int), hipMemcpyHostToDevice, st));
         GF_HIP(hipStreamSynchronize(st));
-        unsigned long long live64[64];
-        GF_HIP(hipMemcpy(live64, d_live.p, sizeof live64, hipMemcpyDeviceToHost));
-        unsigned long long live = 0;
-        for (int k = 0; k < 64; k++) live += live64[k];
-        R->live_mutation_generations = (int64_t)live;
-        float ims = 0;
-        (void)hipEventElapsedTime(&ims, i0, i1);
-        (void)hipEventDestroy(i0);
-        (void)hipEventDestroy(i1);
-        R->timing.init_ms = ims;
-        R->timing.advance_ms = t_adv.total_ms();
-        R->timing.advance_launches = t_adv.count();
-        R->timing.advance_timed_launches = t_adv.timed();
-        R->timing.compact_ms = t_cmp.total_ms();
-        R->timing.compact_launches = t_cmp.count();
-        R->timing.compact_timed_launches = t_cmp.timed();
-        R->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        return GF_OK;
-    }
-
-    /* SFS (and optionally the state) of finished simulations */
-    int finalise(const std::vector<int> &done, DevBuf<unsigned long long> &d_acc, DevBuf<float> &d_lc) {
-        const int nd = (int)done.size();
-        std::vector<int> tseg, tfirst(nd), segM(nd);
-        std::vector<int64_t> sbase(nd);
-        for (int k = 0; k < nd; k++) {
-            const int s = done[k];
-            tfirst[k] = (int)tseg.size();
-            segM[k] = M[s];
-            sbase[k] = base[s];
-            for (int q = 0; q < (M[s] + WG - 1) / WG; q++) tseg.push_back(k);
-        }
-        const int nt = (int)tseg.size();
-        std::vector<int> meta(tseg);
-        meta.insert(meta.end(), tfirst.begin(), tfirst.end());
-        meta.insert(meta.end(), segM.begin(), segM.end());
-        DevBuf<int> d_m;
-        DevBuf<int64_t> d_b;
-        GF_TRY(d_m.alloc(meta.size() + 1));
-        GF_TRY(d_b.alloc(nd));
-        GF_TRY(d_acc.alloc((size_t)nd * (n + 2)));
-        GF_HIP(hipMemcpyAsync(d_m.p, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(d_b.p, sbase.data(), nd * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        GF_TRY(sfs_device(freqA.p, d_b.p, d_m.p + nt + nd, d_m.p, d_m.p + nt, nt, nd, n, d_acc.p, st, d_lc));
-        std::vector<unsigned long long> acc((size_t)nd * (n + 2));
-        GF_HIP(hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        if (keep) {
-            for (int k = 0; k < nd; k++) {
-                const int s = done[k];
-                R->freq[s].resize(M[s]);
-                R->ids[s].resize(M[s]);
-                if (M[s] > 0) {
-                    GF_HIP(hipMemcpyAsync(R->freq[s].data(), freqA.p + base[s], (size_t)M[s] * sizeof(float), hipMemcpyDeviceToHost, st));
-                    GF_HIP(hipMemcpyAsync(R->ids[s].data(), idsA.p + base[s], (size_t)M[s] * sizeof(gf_mutid), hipMemcpyDeviceToHost, st));
+        GF_TRY(capture());
+        unsigned long long live_prev = 0;
+        int per_check = 4;
+        /* every replay advances each active simulation by at least one
+         * generation unless it stalls, and a stall is resolved at the next
+         * check: a bound no progressing batch reaches */
+        const int64_t max_blocks = 4 * (int64_t)maxG + 1024;
+        for (;;) {
+            if ((int64_t)blocks > max_blocks + 64 * (int64_t)(relayouts + 1)) {
+                set_error("gf_batch_run: no progress after " + std::to_string(blocks) + " blocks");
+                return GF_ERR_DEVICE;
+            }
+            GF_HIP(hipEventRecord(w0, st));
+            for (int r = 0; r < per_check; r++) GF_HIP(hipGraphLaunch(exec, st));
+            blocks += per_check;
+            GF_HIP(hipEventRecord(w1, st));
+            GF_HIP(hipEventSynchronize(w1));
+            const int active = h_status.p[0], stalled = h_status.p[1];
+            unsigned long long lv;
+            memcpy(&lv, h_status.p + 2, sizeof lv);
+            if (measuring) {
+                float ms = 0;
+                GF_HIP(hipEventElapsedTime(&ms, w0, w1));
+                BlockTuner::get().record(key, K, (double)(lv - live_prev), ms);
+            } else if (per_check < 32) {
+                per_check *= 2;
+            }
+            if (forced <= 0) {
+                const int nextK = BlockTuner::get().next(key, cap, measuring);
+                if (nextK != K) {
+                    K = nextK;
+                    GF_HIP(hipMemcpyAsync(d_K.p, &K, sizeof(int), hipMemcpyHostToDevice, st));
                 }
             }
+            live_prev = lv;
+            if (active == 0 && stalled == 0) break;
+            if (stalled > 0) {
+                GF_TRY(relayout(K));
+                continue;
+            }
+            /* most tiles belong to finished simulations: drop them */
+            if (!measuring && ntiles > 4096) {
+                GF_TRY(sync_sims());
+                int64_t live_tiles = 0;
+                for (int s : layout) if (host[s].status != BS_DONE) live_tiles += host[s].nt;
+                if (4 * live_tiles < ntiles) GF_TRY(relayout(K));
+            }
         }
+        GF_TRY(sync_sims());
+        std::vector<int> done;
+        for (int s = 0; s < S; s++) if (!finalised[s]) done.push_back(s);
+        GF_TRY(finalise(done));
+        unsigned long long live64[64];
+        GF_HIP(hipMemcpyAsync(live64, live.p, sizeof live64, hipMemcpyDeviceToHost, st));
         GF_HIP(hipStreamSynchronize(st));
-        for (int k = 0; k < nd; k++) {
-            const int s = done[k];
-            R->num_mutations[s] = M[s];
-            finish_sfs(&acc[(size_t)k * (n + 2)], acc[(size_t)k * (n + 2) + n + 1], n, M[s], in[s].num_sites,
-                       &R->sfs[(size_t)s * n]);
-        }
+        unsigned long long lt = 0;
+        for (int k = 0; k < 64; k++) lt += live64[k];
+        R->live_mutation_generations = (int64_t)lt;
+        float ims = 0;
+        (void)hipEventElapsedTime(&ims, i0, i1);
+        R->timing.init_ms = ims;
+        R->timing.advance_launches = blocks;
+        R->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        R->generations_per_block = K;
+        R->blocks = blocks;
+        R->relayouts = relayouts;
         return GF_OK;
     }
 };
@@ -519,8 +1002,13 @@ int32_t gf_batch_run(const gf_batch_sim *sims, int32_t num_sims, int32_t compact
     using namespace gf;
     if (!out) { set_error("gf_batch_run: out is NULL"); return GF_ERR_ARG; }
     *out = nullptr;
-    if (!sims || num_sims <= 0 || sfs_n <= 0 || compact_interval < 0) {
+    if (!sims || num_sims <= 0 || sfs_n <= 0) {
         set_error("gf_batch_run: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    if (compact_interval <= 0) {
+        set_error("gf_batch_run: compact_interval must be > 0 (the batch compacts at every block of at most "
+                  "compact_interval generations)");
         return GF_ERR_ARG;
     }
     GF_TRY(select_device(device));
@@ -566,6 +1054,14 @@ int32_t gf_batch_live_mutation_generations(const gf_batch_result *r, int64_t *ou
 int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out) {
     if (!r || !out) return GF_ERR_ARG;
     *out = r->timing;
+    return GF_OK;
+}
+
+int32_t gf_batch_tuning(const gf_batch_result *r, int32_t *generations_per_block, int32_t *blocks, int32_t *relayouts) {
+    if (!r) return GF_ERR_ARG;
+    if (generations_per_block) *generations_per_block = r->generations_per_block;
+    if (blocks) *blocks = r->blocks;
+    if (relayouts) *relayouts = r->relayouts;
     return GF_OK;
 }
 

@@ -51,6 +51,53 @@ static __global__ __launch_bounds__(HIST_WG) void k_pop_hist(const float *__rest
         if (sh_hist[k]) atomicAdd(&hist[b0 + k], sh_hist[k]);
 }
 
+/* Adds one mutation of frequency pf to the fixed-point level sums of a
+ * sample of n (sh in LDS when LDS, else out in global memory): for
+ * 0 < pf < 1 the terms exp(log C(n,k) + k log p + (n-k) log q), each
+ * truncated to 2^-32 fixed point; a frequency of 0 or 1 counts one
+ * non-segregating mutation at [n + 1]. */
+template <bool LDS>
+__device__ __forceinline__ void sfs_add_mutation(float pf, int n, int half_n, const float *__restrict__ lc,
+                                                 unsigned long long *sh, unsigned long long *out) {
+    if (pf > 0.0f && pf < 1.0f) {
+        const float lp = log_f(pf);
+        const float lq = log_f(1.0f - pf);
+        /* The terms with t >= -23 (the only ones that survive the 2^-32
+         * truncation) form one run of levels around the mode: t is a
+         * concave function of k (log C(n,k) is concave, the rest linear)
+         * evaluated with rounding errors far below 1, so walking out from
+         * the mode until t < -24 on each side visits every one of them, and
+         * adds exactly the terms a loop over all n + 1 levels would. */
+        auto level = [&](int k) {
+            const float coeff = k < half_n ? lc[k] : lc[n - k];
+            return (coeff + (float)k * lp) + (float)(n - k) * lq;
+        };
+        auto add = [&](int k, float t) {
+            if (t < -23.0f) return;
+            const unsigned long long v = (unsigned long long)(exp_f(t) * 4294967296.0f);
+            if (v) {
+                if (LDS) atomicAdd(&sh[k], v);
+                else atomicAdd(&out[k], v);
+            }
+        };
+        int km = (int)((float)(n + 1) * pf);
+        km = km > n ? n : km;
+        for (int k = km; k <= n; k++) {
+            const float t = level(k);
+            if (t < -24.0f) break;
+            add(k, t);
+        }
+        for (int k = km - 1; k >= 0; k--) {
+            const float t = level(k);
+            if (t < -24.0f) break;
+            add(k, t);
+        }
+    } else {
+        if (LDS) atomicAdd(&sh[n + 1], 1ull);
+        else atomicAdd(&out[n + 1], 1ull);
+    }
+}
+
 /* acc layout per segment: [0..n] fixed-point level sums, [n+1] count of
  * mutations that are not segregating (p <= 0 or p >= 1) */
 template <bool LDS>
@@ -67,47 +114,7 @@ __global__ __launch_bounds__(WG) void k_sfs(const float *__restrict__ freq, cons
         for (int k = threadIdx.x; k < n + 2; k += WG) sh[k] = 0;
         __syncthreads();
     }
-    if (li < seg_M[s]) {
-        const float pf = freq[seg_base[s] + li];
-        if (pf > 0.0f && pf < 1.0f) {
-            const float lp = log_f(pf);
-            const float lq = log_f(1.0f - pf);
-            /* The terms with t >= -23 (the only ones that survive the 2^-32
-             * truncation) form one run of levels around the mode: t is a
-             * concave function of k (log C(n,k) is concave, the rest linear)
-             * evaluated with rounding errors far below 1, so walking out
-             * from the mode until t < -24 on each side visits every one of
-             * them, and adds exactly the terms a loop over all n + 1 levels
-             * would. */
-            auto level = [&](int k) {
-                const float coeff = k < half_n ? lc[k] : lc[n - k];
-                return (coeff + (float)k * lp) + (float)(n - k) * lq;
-            };
-            auto add = [&](int k, float t) {
-                if (t < -23.0f) return;
-                const unsigned long long v = (unsigned long long)(exp_f(t) * 4294967296.0f);
-                if (v) {
-                    if (LDS) atomicAdd(&sh[k], v);
-                    else atomicAdd(&out[k], v);
-                }
-            };
-            int km = (int)((float)(n + 1) * pf);
-            km = km > n ? n : km;
-            for (int k = km; k <= n; k++) {
-                const float t = level(k);
-                if (t < -24.0f) break;
-                add(k, t);
-            }
-            for (int k = km - 1; k >= 0; k--) {
-                const float t = level(k);
-                if (t < -24.0f) break;
-                add(k, t);
-            }
-        } else {
-            if (LDS) atomicAdd(&sh[n + 1], 1ull);
-            else atomicAdd(&out[n + 1], 1ull);
-        }
-    }
+    if (li < seg_M[s]) sfs_add_mutation<LDS>(freq[seg_base[s] + li], n, half_n, lc, sh, out);
     if (LDS) {
         __syncthreads();
         for (int k = threadIdx.x; k < n + 2; k += WG)

@@ -160,7 +160,7 @@ def batch_sims(p, seeds):
     return sims
 
 
-def simulate(p, seeds, chunk=8192, compact_interval=16):
+def simulate(p, seeds, chunk=8192, compact_interval=32):
     """expected n=100 sample SFS (bins 1..99) of every draw, by the batched simulator"""
     n = len(seeds)
     out = np.empty((n, NBINS), np.float64)

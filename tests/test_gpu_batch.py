@@ -93,3 +93,54 @@ def test_batch_size_change_matches_plain(gpu):
         assert b["num_mutations"][k] == last["num_mutations"], k
         assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), k
         assert np.array_equal(ids, r["ids"]), k
+
+
+def test_batch_requires_compaction(gpu):
+    """every block of a batch compacts: a compaction interval of 0 (never
+    compact) is refused rather than silently ignored"""
+    sims = make_batch(4, np.random.default_rng(25))
+    with pytest.raises(gpu.GoFishError):
+        gpu.batch_run(sims, compact_interval=0, sfs_n=20)
+
+
+@pytest.mark.parametrize("K", [1, 7, 32])
+def test_batch_block_length_invariance(gpu, K):
+    """the block length (graph replays of K generations, K capped by
+    gf_set_generations_per_launch; otherwise the autotuner's choice) never
+    changes a simulation: every K gives gf_run's states bit for bit"""
+    rng = np.random.default_rng(26)
+    sims = make_batch(16, rng, G_scale=0.5)
+    gpu.set_generations_per_launch(K)
+    try:
+        b = gpu.batch_run(sims, compact_interval=35, sfs_n=30, keep_states=True)
+    finally:
+        gpu.set_generations_per_launch(0)
+    assert b["tuning"]["generations_per_block"] == K
+    for k, sim in enumerate(sims):
+        r = gpu.run(plain_desc(sim, 35))
+        last = r["samples"][-1]
+        f, ids = b["states"][k]
+        assert b["num_mutations"][k] == last["num_mutations"], (K, k)
+        assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), (K, k)
+        assert np.array_equal(ids, r["ids"]), (K, k)
+
+
+def test_batch_growth_relayouts(gpu):
+    """populations that grow 20-80 fold after generation 0 outgrow the tiles
+    their equilibrium start was given: the batch stalls them, lays the live
+    simulations out again (at least once here) and goes on; every result
+    still equals gf_run's"""
+    rng = np.random.default_rng(27)
+    sims = make_batch(20, rng, G_scale=0.6)
+    sims["N1"] = (sims["N"] * rng.uniform(20, 80, len(sims))).astype(np.int32)
+    b = gpu.batch_run(sims, compact_interval=16, sfs_n=40, keep_states=True)
+    assert b["tuning"]["relayouts"] >= 1
+    for k, sim in enumerate(sims):
+        d = plain_desc(sim, 35)
+        d.params["N"][1:] = sim["N1"]
+        r = gpu.run(d)
+        last = r["samples"][-1]
+        f, ids = b["states"][k]
+        assert b["num_mutations"][k] == last["num_mutations"], k
+        assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), k
+        assert np.array_equal(ids, r["ids"]), k

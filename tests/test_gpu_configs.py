@@ -119,7 +119,7 @@ def test_batched_sweep_full_size_bit_identical_to_plain(gpu):
     gf_run has the batch's SFS and final mutation count bit for bit"""
     sims = S.parameter_sweep(4096)
     print("batched sweep 4096", flush=True)
-    b = gpu.batch_run(sims, compact_interval=16, sfs_n=50)
+    b = gpu.batch_run(sims, compact_interval=32, sfs_n=50)
     assert b["sfs"].shape == (4096, 50) and np.all(np.isfinite(b["sfs"]))
     Nc = np.floor(2.0 * sims["N"] / (1.0 + sims["F"]))
     work = sims["num_sites"] * 2e-9 * Nc * S.harmonic(Nc - 1) * sims["num_generations"]
