@@ -247,6 +247,10 @@ int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out);
  * and the number of re-layouts (simulations that outgrew their tiles) */
 int32_t gf_batch_tuning(const gf_batch_result *r, int32_t *generations_per_block, int32_t *blocks,
                         int32_t *relayouts);
+/* Scales the tiles each simulation of a batch is first given (default 1;
+ * (0, 1]): smaller regions make simulations outgrow them, stall and be laid
+ * out again sooner; results are identical with any scale. */
+int32_t gf_set_batch_region_scale(float scale);
 void    gf_batch_free(gf_batch_result *r);
 
 /* ----- learned surrogate of the sample SFS (no reference counterpart) -----

@@ -88,6 +88,7 @@ SIGNATURES = {
     "gf_batch_live_mutation_generations": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     "gf_batch_timing": (C.c_int32, [C.c_void_p, C.POINTER(gf_timing)]),
     "gf_batch_tuning": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gf_set_batch_region_scale": (C.c_int32, [C.c_float]),
     "gf_batch_free": (None, [C.c_void_p]),
     "gf_run_with_selection": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gf_result_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p]),
@@ -312,6 +313,10 @@ def set_kernel_timing(on):
 
 def set_generations_per_launch(k):
     check(load().gf_set_generations_per_launch(int(k)))
+
+
+def set_batch_region_scale(scale):
+    check(load().gf_set_batch_region_scale(float(scale)))
 
 
 MATH_OPS = dict(philox=0, u01=1, exp=2, log=3, normcdfinv=4, binom_draw=5, pois_draw=6, sqrt=7, div=8,

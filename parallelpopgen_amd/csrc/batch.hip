@@ -70,6 +70,7 @@ struct gf_batch_result {
 namespace gf {
 
 constexpr int BK_MAX = 32;                   /* longest block the autotuner tries */
+static float g_region_scale = 1.0f;          /* gf_set_batch_region_scale */
 enum { BS_ACTIVE = 0, BS_DONE = 1, BS_STALLED = 2 };
 
 /* one simulation of a batch, device-resident */
@@ -89,6 +90,7 @@ struct alignas(16) BSim {
     int status;   /* BS_* */
     /* this block (k_bt_rows) */
     int Kb, nnew, tiles;
+    int need;     /* tiles a stalled simulation's block needs */
 };
 
 /* device pointers of a layout (kernel argument, fixed while a graph lives) */
@@ -151,6 +153,7 @@ static __global__ void k_bt_rows(BSim *__restrict__ sims, BBufs B) {
     const int tiles = (int)((need + WG - 1) / WG);
     if (tiles > b.nt) {   /* does not fit its region: wait for the host */
         b.status = BS_STALLED;
+        b.need = tiles;
         return;
     }
     b.Kb = K;
@@ -645,7 +648,8 @@ struct BatchRun {
             /* a population that grows after generation 0 keeps more of its
              * new mutations for a while: room for ~4 ln(G) generations more */
             const int grow = d.N > in[s].N ? (int)std::ceil(4.0 * std::log1p((double)d.G)) : 0;
-            d.nt = std::max(tiles_for(d.M, lam, Kmax + grow), (int)((d.M + WG - 1) / WG));
+            const int nt = (int)std::ceil((double)g_region_scale * tiles_for(d.M, lam, Kmax + grow));
+            d.nt = std::max(nt, (int)((d.M + WG - 1) / WG));
             t += d.nt;
             pcount++;
             layout.push_back(s);
@@ -832,7 +836,8 @@ struct BatchRun {
             const double lam = (double)b.mu * (double)b.Nc * (double)b.L;
             int nt = b.nt;
             if (b.status == BS_STALLED) {
-                nt = std::max(2 * b.nt, tiles_for(b.M, lam, K));
+                const int sized = (int)std::ceil((double)g_region_scale * tiles_for(b.M, lam, K));
+                nt = std::max({2 * b.nt, b.need, sized});
                 b.status = BS_ACTIVE;
             }
             b.nt = nt;
@@ -1054,6 +1059,12 @@ int32_t gf_batch_live_mutation_generations(const gf_batch_result *r, int64_t *ou
 int32_t gf_batch_timing(const gf_batch_result *r, gf_timing *out) {
     if (!r || !out) return GF_ERR_ARG;
     *out = r->timing;
+    return GF_OK;
+}
+
+int32_t gf_set_batch_region_scale(float scale) {
+    if (!(scale > 0.0f && scale <= 1.0f)) { gf::set_error("gf_set_batch_region_scale: scale must be in (0, 1]"); return GF_ERR_ARG; }
+    gf::g_region_scale = scale;
     return GF_OK;
 }
 

@@ -65,6 +65,14 @@ def test_dadi_scenario_matches_reference_runs(L, key):
     assert len(m) == len(rm) == 1000
     z = (m - rm) / np.sqrt(s ** 2 / 50 + rs ** 2 / 50)
     assert np.all(np.abs(z) < 4.5), (np.argmax(np.abs(z)) + 1, z[np.argmax(np.abs(z))])
+    # Only an upper bound on the mean squared z: the 1,000 bins of one
+    # 50-run mean are strongly positively correlated (one run's extra
+    # mutations raise all its normalised bins together, and the same 50 runs
+    # feed every bin), so mean(z^2) over bins does not concentrate at 1 --
+    # it scatters like a chi-square with few effective degrees of freedom,
+    # well below 1 as often as above (0.48 in one run of the L = 2e8 case).
+    # Too large a value still flags a biased model; too small one flags
+    # nothing.
     assert np.mean(z ** 2) < 1.6, np.mean(z ** 2)
     nm = tail["average number of mutations in SFS"]
     assert abs(nm / ref["average_number_of_mutations_in_SFS"] - 1) < 0.01, nm

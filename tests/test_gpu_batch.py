@@ -125,16 +125,23 @@ def test_batch_block_length_invariance(gpu, K):
         assert np.array_equal(ids, r["ids"]), (K, k)
 
 
-def test_batch_growth_relayouts(gpu):
-    """populations that grow 20-80 fold after generation 0 outgrow the tiles
-    their equilibrium start was given: the batch stalls them, lays the live
-    simulations out again (at least once here) and goes on; every result
-    still equals gf_run's"""
+@pytest.mark.parametrize("scale", [1.0, 0.02])
+def test_batch_growth_relayouts(gpu, scale):
+    """populations that grow 20-80 fold after generation 0 keep more
+    mutations than their equilibrium start: with regions cut to 2 % of the
+    library's sizing they outgrow their tiles, stall, and are laid out again
+    (several times) while the others go on; every result still equals
+    gf_run's, with the default sizing too"""
     rng = np.random.default_rng(27)
     sims = make_batch(20, rng, G_scale=0.6)
     sims["N1"] = (sims["N"] * rng.uniform(20, 80, len(sims))).astype(np.int32)
-    b = gpu.batch_run(sims, compact_interval=16, sfs_n=40, keep_states=True)
-    assert b["tuning"]["relayouts"] >= 1
+    gpu.set_batch_region_scale(scale)
+    try:
+        b = gpu.batch_run(sims, compact_interval=16, sfs_n=40, keep_states=True)
+    finally:
+        gpu.set_batch_region_scale(1.0)
+    if scale < 1:
+        assert b["tuning"]["relayouts"] >= 2
     for k, sim in enumerate(sims):
         d = plain_desc(sim, 35)
         d.params["N"][1:] = sim["N1"]
