@@ -40,7 +40,8 @@ static __global__ __launch_bounds__(HIST_WG) void k_pop_hist(const float *__rest
     const int nb = Nc - b0 < chunk ? Nc - b0 : chunk;
     for (int k = threadIdx.x; k < nb; k += HIST_WG) sh_hist[k] = 0;
     __syncthreads();
-    for (int i = blockIdx.x * HIST_WG + threadIdx.x; i < M; i += gridDim.x * HIST_WG) {
+    /* 64-bit index: i + stride must not wrap for M near INT_MAX */
+    for (int64_t i = (int64_t)blockIdx.x * HIST_WG + threadIdx.x; i < M; i += (int64_t)gridDim.x * HIST_WG) {
         int idx = f2i_sat(round_away((float)Nc * freq[i]));
         if (idx == Nc) idx = 0;
         const unsigned int r = (unsigned int)(idx - b0);
