@@ -182,6 +182,20 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t num_mutations,
                                    int32_t sample_size, float num_sites,
                                    float *out, int32_t device);
 
+/* gf_site_frequency_spectrum on a row already in device memory, with the
+ * spectrum written to device memory, that allocates nothing (no
+ * counterpart in the reference: it is for callers that cannot allocate --
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d_freq: num_mutations floats and d_out: n floats, both on
+ * the current device; the kernels run on `stream` (a hipStream_t; NULL is
+ * the null stream) and the call returns without waiting for them.  Its
+ * scratch is one fixed device array per device (656 KiB), shared by every
+ * call on that device: calls that overlap on one device race.  n is at most
+ * 65,536 (GF_ERR_ARG beyond).  Same result, bit for bit, as
+ * gf_site_frequency_spectrum. */
+int32_t gf_site_frequency_spectrum_noalloc(const float *d_freq, int32_t num_mutations, int32_t n,
+                                           float num_sites, float *d_out, void *stream);
+
 /* The same spectrum for time sample `sample`, population `population` of a
  * gf_run result, computed where the sample lives: the final sample stays
  * in device memory after gf_run, so it is binned there without a copy to

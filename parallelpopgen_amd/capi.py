@@ -78,6 +78,8 @@ SIGNATURES = {
     "gf_result_free": (None, [C.c_void_p]),
     "gf_check_sim": (C.c_int32, [C.POINTER(gf_sim_desc)]),
     "gf_release_device_cache": (C.c_int32, [C.c_int32]),
+    "gf_site_frequency_spectrum_noalloc": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
+                                                      C.c_void_p]),
     "gf_population_histogram": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_site_frequency_spectrum": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
     "gf_batch_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,

@@ -123,6 +123,32 @@ __global__ __launch_bounds__(WG) void k_sfs(const float *__restrict__ freq, cons
     }
 }
 
+/* The sample SFS of one device row with every step on the device (no host
+ * copy, no allocation): k_sfs_prep writes the log-binomial table lc[0 ..
+ * half_n) with the host table's sequential float sum (log_binom_table) and
+ * clears acc[0 .. n + 1]; k_sfs_row accumulates, a grid-stride loop over
+ * the M mutations; k_sfs_finish converts to the float spectrum as
+ * finish_sfs does. */
+__global__ void k_sfs_prep(int n, int half_n, float *__restrict__ lc, unsigned long long *__restrict__ acc);
+template <bool LDS>
+__global__ __launch_bounds__(WG) void k_sfs_row(const float *__restrict__ freq, int M, int n, int half_n,
+                                                const float *__restrict__ lc, unsigned long long *__restrict__ acc) {
+    __shared__ unsigned long long sh[LDS ? SFS_LDS_BINS : 1];
+    if (LDS) {
+        for (int k = threadIdx.x; k < n + 2; k += WG) sh[k] = 0;
+        __syncthreads();
+    }
+    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < M; i += (int64_t)gridDim.x * WG)
+        sfs_add_mutation<LDS>(freq[i], n, half_n, lc, sh, acc);
+    if (LDS) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < n + 2; k += WG)
+            if (sh[k]) atomicAdd(&acc[k], sh[k]);
+    }
+}
+__global__ void k_sfs_finish(const unsigned long long *__restrict__ acc, int n, int M, float num_sites,
+                             float *__restrict__ out);
+
 void log_binom_table(int n, std::vector<float> &lc, int &half_n);
 void finish_sfs(const unsigned long long *acc, unsigned long long nonseg, int n, int M, float num_sites, float *out);
 /* sample SFS of one population row already in device memory (M floats) */

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace gf {
@@ -88,6 +90,34 @@ int joint_sfs_of_device_rows(const float *d_fa, const float *d_fb, int M, int na
     return GF_OK;
 }
 
+__global__ void k_sfs_prep(int n, int half_n, float *__restrict__ lc, unsigned long long *__restrict__ acc) {
+    GF_STRICT_FP
+    for (int k = threadIdx.x; k < n + 2; k += blockDim.x) acc[k] = 0ull;
+    for (int k = 1 + (int)threadIdx.x; k < half_n; k += blockDim.x)
+        lc[k] = log_f(((float)n + 1.0f - (float)k) / (float)k);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = 0.0f;
+        lc[0] = 0.0f;
+        for (int k = 1; k < half_n; k++) {
+            a = a + lc[k];
+            lc[k] = a;
+        }
+    }
+}
+
+__global__ void k_sfs_finish(const unsigned long long *__restrict__ acc, int n, int M, float num_sites,
+                             float *__restrict__ out) {
+    GF_STRICT_FP
+    const double sc = 1.0 / 4294967296.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        if (k == 0)
+            out[0] = (float)((double)(acc[0] + acc[n]) * sc + (double)acc[n + 1] + ((double)num_sites - (double)M));
+        else
+            out[k] = (float)((double)acc[k] * sc);
+    }
+}
+
 /* sample SFS of one population row already in device memory (M floats) */
 int sfs_of_device_row(const float *d_row, int M, int n, float num_sites, float *out, hipStream_t st,
                       SfsScratch *scratch) {
@@ -138,6 +168,47 @@ static hipError_t launch_pop_hist(const float *d_freq, int M, int Nc, unsigned i
     hipLaunchKernelGGL(k_pop_hist, dim3(strips, ranges), dim3(HIST_WG), (size_t)chunk * sizeof(unsigned int), st,
                        d_freq, M, Nc, chunk, d_hist);
     return hipGetLastError();
+}
+
+/* ---------------- the sample SFS without allocation ----------------
+ * Scratch of gf_site_frequency_spectrum_noalloc: the fixed-point level sums
+ * (n + 2 words of 64 bits) and the log-binomial table (n/2 + 1 floats) of
+ * one call, for sample sizes up to SFS_NOALLOC_MAX_N: 656 KiB in a
+ * file-scope device array, one copy on every device the library runs on,
+ * held until the process ends.  Host code never passes the array's name to
+ * a kernel or copy (there it only stands for the device symbol): it takes
+ * the device address from hipGetSymbolAddress, once per device with that
+ * device current, and uses that pointer. */
+constexpr int SFS_NOALLOC_MAX_N = 65536;
+constexpr size_t SFS_NOALLOC_ACC_WORDS = (size_t)SFS_NOALLOC_MAX_N + 2;
+constexpr size_t SFS_NOALLOC_LC_FLOATS = (size_t)SFS_NOALLOC_MAX_N / 2 + 1;
+constexpr size_t SFS_NOALLOC_BYTES = SFS_NOALLOC_ACC_WORDS * sizeof(unsigned long long) +
+                                     SFS_NOALLOC_LC_FLOATS * sizeof(float);
+static __device__ __align__(256) unsigned char scratch[SFS_NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES];
+
+/* device address of `scratch` on the current device `dev` */
+static int scratch_on_device(int dev, void **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("site_frequency_spectrum_noalloc: device index beyond " + std::to_string(MAX_DEVICES - 1));
+        return GF_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("site_frequency_spectrum_noalloc: hipGetSymbolAddress failed: ") +
+                      hipGetErrorString(e));
+            return GF_ERR_DEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GF_OK;
 }
 
 } /* namespace gf */
@@ -205,6 +276,38 @@ int32_t gf_site_frequency_spectrum(const float *freq, int32_t M, int32_t n, floa
         if (!rc) rc = sfs_of_device_row(d_f.p, M, n, num_sites, out, st);
     }
     return rc;
+}
+
+int32_t gf_site_frequency_spectrum_noalloc(const float *d_freq, int32_t M, int32_t n, float num_sites, float *d_out,
+                                           void *stream) {
+    if (!d_out || n <= 0 || M < 0 || (M > 0 && !d_freq)) {
+        set_error("site_frequency_spectrum_noalloc: invalid arguments");
+        return GF_ERR_ARG;
+    }
+    if (n > SFS_NOALLOC_MAX_N) {
+        set_error("site_frequency_spectrum_noalloc: sample size " + std::to_string(n) + " above the maximum " +
+                  std::to_string(SFS_NOALLOC_MAX_N) + " of the allocation-free entry point");
+        return GF_ERR_ARG;
+    }
+    int dev = 0;
+    GF_HIP(hipGetDevice(&dev));
+    void *base = nullptr;
+    GF_TRY(scratch_on_device(dev, &base));
+    unsigned long long *acc = (unsigned long long *)base;
+    float *lc = (float *)(acc + SFS_NOALLOC_ACC_WORDS);
+    const int half_n = (n % 2 == 0) ? n / 2 + 1 : (n + 1) / 2;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_sfs_prep, dim3(1), dim3(WG), 0, st, n, half_n, lc, acc);
+    if (M > 0) {
+        const int grid = std::max(1, std::min((M + WG - 1) / WG, num_cus() * 8));
+        if (n + 2 <= SFS_LDS_BINS)
+            hipLaunchKernelGGL((k_sfs_row<true>), dim3(grid), dim3(WG), 0, st, d_freq, M, n, half_n, lc, acc);
+        else
+            hipLaunchKernelGGL((k_sfs_row<false>), dim3(grid), dim3(WG), 0, st, d_freq, M, n, half_n, lc, acc);
+    }
+    hipLaunchKernelGGL(k_sfs_finish, dim3(1), dim3(WG), 0, st, acc, n, M, num_sites, d_out);
+    GF_HIP(hipGetLastError());
+    return GF_OK;
 }
 
 int32_t gf_joint_site_frequency_spectrum(const float *freq_a, const float *freq_b, int32_t M, int32_t n_a, int32_t n_b,
