@@ -14,11 +14,18 @@ BUILD = os.path.join(REPO, "examples", "_build")
 GOLDEN = os.path.join(REPO, "tests", "golden")
 
 
+# the reference's own example_compilation sources (compiled unchanged where
+# the reference checkout is present; the binary travels with the build)
+REF_EXAMPLE = "/root/reference/examples/example_compilation"
+
+
 def test_examples_build():
     r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "examples")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
     for exe in ("speed", "dadi", "compilation", "custom"):
         assert os.access(os.path.join(BUILD, exe), os.X_OK)
+    if os.path.exists(os.path.join(REF_EXAMPLE, "run.cu")):
+        assert os.access(os.path.join(BUILD, "ref_compilation"), os.X_OK)
 
 
 def run_example(name, *args, timeout=600):
@@ -87,9 +94,8 @@ def test_speed_example_output():
     assert abs(n - np.mean(vals)) < 4.5 * np.sqrt(np.mean(vals)) * 1.25, (n, vals)
 
 
-@pytest.mark.gpu
-def test_mixed_compiler_example(tmp_path):
-    out = run_example("compilation", str(tmp_path / "bfile.dat"))
+def check_compilation_output(out, bfile):
+    """the example_compilation scenario's printed summary and its bfile.dat"""
     lines = out.splitlines()
     ns = int([l for l in lines if l.startswith("number of time samples")][0].split(":")[1])
     m0 = int([l for l in lines if l.startswith("mutations in first time sample")][0].split(":")[1])
@@ -97,7 +103,7 @@ def test_mixed_compiler_example(tmp_path):
     assert ns == 2
     # reference example_compilation.out: 5288 and 16736 (a single run)
     assert 0.7 * 5288 < m0 < 1.3 * 5288 and 0.7 * 16736 < m1 < 1.3 * 16736, (m0, m1)
-    text = open(tmp_path / "bfile.dat").read().splitlines()
+    text = open(bfile).read().splitlines()
     assert text[0].startswith("seed1\t-1091571699") and text[1].startswith("seed2\t-557122643")
     assert text[10].startswith("time sample:") and text[11].split("\t")[1:3] == ["20000", ""]
     rows = [l for l in text if l.startswith("(")]
@@ -105,6 +111,31 @@ def test_mixed_compiler_example(tmp_path):
     freqs = np.array([[float(x) for x in r.split("\t")[1:]] for r in rows[:500]])
     assert freqs.shape[1] == 4
     assert np.allclose(np.round(freqs * 1000), freqs * 1000, atol=1e-3)   # 1000 haploid chromosomes
+
+
+@pytest.mark.gpu
+def test_mixed_compiler_example(tmp_path):
+    out = run_example("compilation", str(tmp_path / "bfile.dat"))
+    check_compilation_output(out, tmp_path / "bfile.dat")
+
+
+@pytest.mark.gpu
+def test_reference_compilation_example_unchanged(tmp_path):
+    """The reference's examples/example_compilation (run.cu through hipcc as
+    HIP, main.cpp through g++, neither file changed) built against include/
+    and libgofish.so: it runs and prints what the retyped copy prints.  It
+    writes bfile.dat to its working directory."""
+    exe = os.path.join(BUILD, "ref_compilation")
+    if not os.path.exists(exe):
+        pytest.skip("built only where the reference's example sources are present (make -C examples)")
+    newest = max(os.path.getmtime(f) for f in
+                 [os.path.join(REPO, "parallelpopgen_amd", "libgofish.so")] +
+                 [os.path.join(REPO, "include", x) for x in os.listdir(os.path.join(REPO, "include"))])
+    if os.path.getmtime(exe) < newest:
+        pytest.fail("ref_compilation is older than the library or its headers (make -C examples)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    check_compilation_output(r.stdout, tmp_path / "bfile.dat")
 
 
 @pytest.mark.gpu
