@@ -7,10 +7,11 @@
  * (99 softmax logits for the n = 100 SFS bins 1..99, one log-total head),
  * BF16 weights and activations, FP32 accumulation, trained by this file's
  * forward, backward and Adam kernels.  Every matrix product runs on the
- * gfx950 matrix cores (v_mfma_f32_32x32x16_bf16) through one tiled kernel
- * whose global->LDS stage handles the three operand layouts of
- * forward (X W^T), input gradient (dZ W) and weight gradient (dZ^T X);
- * the ensemble members are the grid's z dimension.
+ * gfx950 matrix cores (v_mfma_f32_32x32x16_bf16): k_gemm's global->LDS
+ * stage handles the three operand layouts of forward (X W^T), input
+ * gradient (dZ W) and weight gradient (dZ^T X) at training batch sizes;
+ * k_gemm_nt runs the forward products of large batches (prediction); the
+ * ensemble members are the grid's z dimension.
  */
 #include "gf_runtime.hpp"
 
@@ -53,17 +54,25 @@ struct Layout {
     }
 };
 
-__device__ __forceinline__ float gelu(float x) {
-    const float c = 0.7978845608028654f;   /* sqrt(2/pi) */
-    return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
-}
-
-__device__ __forceinline__ float gelu_grad(float x) {
+/* GELU, tanh form: 0.5 x (1 + tanh(u)), u = c (x + 0.044715 x^3), c =
+ * sqrt(2/pi).  With sigma(v) = 1 / (1 + e^-v), 0.5 (1 + tanh(u)) =
+ * sigma(2u), so gelu(x) = x sigma(2u), evaluated with the hardware exp2 and
+ * reciprocal (about 1e-6 relative; the activations are then rounded to
+ * bfloat16).  e^-2u overflows to inf for u << 0, where sigma -> 0. */
+__device__ __forceinline__ float sigma2u(float x) {
     const float c = 0.7978845608028654f;
     const float u = c * (x + 0.044715f * x * x * x);
-    const float t = tanhf(u);
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+}
+
+__device__ __forceinline__ float gelu(float x) { return x * sigma2u(x); }
+
+/* d/dx x sigma(2u) = sigma + 2 x sigma (1 - sigma) u' */
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float c = 0.7978845608028654f;
+    const float sg = sigma2u(x);
     const float du = c * (1.0f + 3.0f * 0.044715f * x * x);
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
+    return sg + 2.0f * x * sg * (1.0f - sg) * du;
 }
 
 enum Epi { EPI_HIDDEN = 0, EPI_OUTPUT = 1, EPI_DX = 2, EPI_DW = 3 };
@@ -140,6 +149,120 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
             g.Cb[z * g.sCb + o] = (bf16)(v * gelu_grad(g.Zprev[z * g.sZ + o]));
         } else {
             g.Cf[z * g.sCf + o] = v;
+        }
+    }
+}
+
+/* ---- large forward products: C = A B^T with both operands K-contiguous ----
+ * (the forward pass and prediction at batches of thousands of rows).  A
+ * 128 x 128 tile per workgroup of 4 waves, each wave a 64 x 64 quadrant of
+ * 2 x 2 blocks of v_mfma_f32_32x32x16_bf16; K in stages of 64 through two
+ * LDS buffers: the next stage's global loads are issued before the current
+ * stage's MFMAs and stored to the other buffer after them, one barrier per
+ * stage.  Each output element accumulates its k = 0, 16, 32, ... products
+ * in the same order as k_gemm, so both kernels give the same bits.
+ *
+ * Tiles are dealt XCD-aware: the hardware hands workgroup b to XCD b % 8,
+ * so tile L is given to the workgroup that puts the tiles of one stretch of
+ * L (the N tiles of an M tile, fastest) on one XCD, where they share the A
+ * rows in that XCD's L2. */
+constexpr int GT = 128;
+constexpr int GK = 64;
+constexpr int GLDS = GK + 8;   /* padded LDS row: conflict-free 16-byte fragment reads */
+constexpr int GR = GK / 8;     /* threads per tile row in the global -> LDS copy */
+constexpr int GRS = 256 / GR;  /* rows one pass of the workgroup copies */
+constexpr int GP = GT / GRS;   /* passes per operand and stage */
+
+__global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs g, int MT, int NT, int T) {
+    __shared__ bf16 As[2][GT][GLDS];
+    __shared__ bf16 Bs[2][GT][GLDS];
+    const int b = (int)blockIdx.x;
+    const int xcd = b & 7, r = b >> 3;
+    const int per = T >> 3, rem = T & 7;
+    const int L = xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+    const int nt = L % NT, mt = (L / NT) % MT, z = L / (NT * MT);
+    const bf16 *A = g.A + z * g.sA;
+    const bf16 *B = g.B + z * g.sB;
+    const int m0 = mt * GT, n0 = nt * GT;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    /* global -> LDS: each thread moves rows lr + GRS i (i < GP), 8 bf16 at column lk */
+    const int lr = t / GR, lk = (t % GR) * 8;
+    const bf16 *pa = A + (int64_t)(m0 + lr) * g.lda + lk;
+    const bf16 *pb = B + (int64_t)(n0 + lr) * g.ldb + lk;
+    const int64_t da = GRS * g.lda, db = GRS * g.ldb;
+    bf16x8 ra[GP], rb[GP];
+#pragma unroll
+    for (int i = 0; i < GP; i++) {
+        ra[i] = *(const bf16x8 *)(pa + i * da);
+        rb[i] = *(const bf16x8 *)(pb + i * db);
+    }
+#pragma unroll
+    for (int i = 0; i < GP; i++) {
+        *(bf16x8 *)&As[0][lr + GRS * i][lk] = ra[i];
+        *(bf16x8 *)&Bs[0][lr + GRS * i][lk] = rb[i];
+    }
+    __syncthreads();
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x16{};
+    const int ns = g.K / GK;
+    const int fr = lane & 31, fk = 8 * (lane >> 5);
+    for (int s = 0; s < ns; s++) {
+        const int cur = s & 1;
+        const bool more = s + 1 < ns;
+        if (more) {
+            const int k1 = (s + 1) * GK;
+#pragma unroll
+            for (int i = 0; i < GP; i++) {
+                ra[i] = *(const bf16x8 *)(pa + i * da + k1);
+                rb[i] = *(const bf16x8 *)(pb + i * db + k1);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 16) {
+            bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8 *)&As[cur][wm * 64 + i * 32 + fr][kk + fk];
+#pragma unroll
+            for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8 *)&Bs[cur][wn * 64 + j * 32 + fr][kk + fk];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            /* nobody reads buffer cur ^ 1 in this stage: it was last read
+             * in stage s - 1, before the barrier that ended it */
+#pragma unroll
+            for (int i = 0; i < GP; i++) {
+                *(bf16x8 *)&As[cur ^ 1][lr + GRS * i][lk] = ra[i];
+                *(bf16x8 *)&Bs[cur ^ 1][lr + GRS * i][lk] = rb[i];
+            }
+        }
+        __syncthreads();
+    }
+    /* epilogue: accumulator r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32] of its block */
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int n = n0 + wn * 64 + j * 32 + fr;
+        const float bias = g.bias ? g.bias[z * g.sBias + n] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                const int64_t o = (int64_t)m * g.ldc + n;
+                const float v = acc[i][j][q] + bias;
+                if (g.epi == EPI_HIDDEN) {
+                    if (g.Cf) g.Cf[z * g.sCf + o] = v;
+                    g.Cb[z * g.sCb + o] = (bf16)gelu(v);
+                } else {
+                    g.Cf[z * g.sCf + o] = v;
+                }
+            }
         }
     }
 }
@@ -248,36 +371,51 @@ __global__ void k_pack_inputs(const float *__restrict__ x, int B, int nin, bf16 
 
 /* ensemble statistics of the predictions: per-model softmax, mean SFS,
  * mean log-total and the ensemble disagreement (largest relative standard
- * deviation across models over the bins with mean mass >= 1e-3) */
-__global__ void k_predict_stats(const float *__restrict__ logits, int64_t sL, int E, int B,
-                                float *__restrict__ sfs, float *__restrict__ tot, float *__restrict__ dis) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    float mean[NBINS], m2[NBINS];
-    for (int k = 0; k < NBINS; k++) { mean[k] = 0.f; m2[k] = 0.f; }
-    float tsum = 0.f;
+ * deviation across models over the bins with mean mass >= 1e-3).  One wave
+ * per input: lane l holds bins l and l + 64, so each model's logit row is
+ * read once, coalesced, and the row's max and sum are wave reductions. */
+__global__ __launch_bounds__(256) void k_predict_stats(const float *__restrict__ logits, int64_t sL, int E, int B,
+                                                       float *__restrict__ sfs, float *__restrict__ tot,
+                                                       float *__restrict__ dis) {
+    const int lane = threadIdx.x & 63;
+    const int b = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (b >= B) return;   /* the whole wave */
+    const int k1 = lane + 64;
+    const bool h1 = k1 < NBINS;
+    float mean0 = 0.f, mean1 = 0.f, sq0 = 0.f, sq1 = 0.f, tsum = 0.f;
     for (int z = 0; z < E; z++) {
         const float *L = logits + z * sL + (int64_t)b * OUT_PAD;
-        float mx = L[0];
-        for (int k = 1; k < NBINS; k++) mx = fmaxf(mx, L[k]);
-        float s = 0.f;
-        for (int k = 0; k < NBINS; k++) s += __expf(L[k] - mx);
-        for (int k = 0; k < NBINS; k++) {
-            const float p = __expf(L[k] - mx) / s;
-            mean[k] += p;
-            m2[k] += p * p;
-        }
+        const float l0 = L[lane];
+        const float l1 = h1 ? L[k1] : l0;
+        float mx = fmaxf(l0, l1);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        const float e0 = __expf(l0 - mx), e1 = h1 ? __expf(l1 - mx) : 0.f;
+        float sum = e0 + e1;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        const float p0 = e0 / sum, p1 = e1 / sum;
+        mean0 += p0; sq0 += p0 * p0;
+        mean1 += p1; sq1 += p1 * p1;
         tsum += L[NBINS];
     }
     float worst = 0.f;
-    for (int k = 0; k < NBINS; k++) {
-        const float mu = mean[k] / E;
-        const float var = fmaxf(m2[k] / E - mu * mu, 0.f);
-        sfs[(int64_t)b * NBINS + k] = mu;
+    {
+        const float mu = mean0 / E, var = fmaxf(sq0 / E - mu * mu, 0.f);
+        sfs[(int64_t)b * NBINS + lane] = mu;
+        if (mu >= 1e-3f) worst = sqrtf(var) / mu;
+    }
+    if (h1) {
+        const float mu = mean1 / E, var = fmaxf(sq1 / E - mu * mu, 0.f);
+        sfs[(int64_t)b * NBINS + k1] = mu;
         if (mu >= 1e-3f) worst = fmaxf(worst, sqrtf(var) / mu);
     }
-    tot[b] = tsum / E;
-    dis[b] = worst;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) worst = fmaxf(worst, __shfl_xor(worst, off));
+    if (lane == 0) {
+        tot[b] = tsum / E;
+        dis[b] = worst;
+    }
 }
 
 } /* namespace sg */
@@ -290,6 +428,7 @@ struct gf_sg_model {
     int E = 4, Bmax = 0;
     Layout lay;
     OwnedStream st;   /* before the buffers: they are released while it is alive */
+    OwnedStream cst;  /* prediction read-back, beside the next batch's compute on st */
     DevBuf<float> w, g, m, v;       /* [E][total] */
     DevBuf<bf16> wb;
     DevBuf<bf16> x;                 /* [B][64] */
@@ -299,7 +438,7 @@ struct gf_sg_model {
     DevBuf<bf16> dlog;              /* [E][B][128] */
     DevBuf<bf16> dz0, dz1;          /* [E][B][512] */
     DevBuf<float> y, tot, loss, xin;
-    DevBuf<float> psfs, ptot, pdis;
+    DevBuf<float> psfs, ptot, pdis; /* two batches' prediction outputs */
     int64_t step = 0;
     float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
 };
@@ -309,7 +448,14 @@ namespace sg {
 
 static int gemm(gf_sg_model &S, GemmArgs a) {
     if (a.M % TILE || a.N % TILE || a.K % KSTEP) { set_error("surrogate gemm: shape not a multiple of the tile"); return GF_ERR_ARG; }
-    hipLaunchKernelGGL(k_gemm, dim3(a.N / TILE, a.M / TILE, S.E), dim3(256), 0, S.st, a);
+    const bool nt = !a.a_km && a.b_nk && (a.epi == EPI_HIDDEN || a.epi == EPI_OUTPUT);
+    if (nt && a.M >= 4096 && a.M % GT == 0 && a.N % GT == 0 && a.K % GK == 0) {
+        const int MT = a.M / GT, NT = a.N / GT, T = MT * NT * S.E;
+        hipLaunchKernelGGL(k_gemm_nt, dim3(T), dim3(256), 0, S.st, a, MT, NT, T);
+    } else {
+        if (a.epi == EPI_HIDDEN && !a.Cf) { set_error("surrogate gemm: small products keep Z"); return GF_ERR_ARG; }
+        hipLaunchKernelGGL(k_gemm, dim3(a.N / TILE, a.M / TILE, S.E), dim3(256), 0, S.st, a);
+    }
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
@@ -327,15 +473,17 @@ static int ensure_batch(gf_sg_model &S, int B) {
     GF_TRY(S.y.alloc((size_t)B * NBINS));
     GF_TRY(S.tot.alloc(B));
     GF_TRY(S.xin.alloc((size_t)B * 6));
-    GF_TRY(S.psfs.alloc((size_t)B * NBINS));
-    GF_TRY(S.ptot.alloc(B));
-    GF_TRY(S.pdis.alloc(B));
+    GF_TRY(S.psfs.alloc((size_t)2 * B * NBINS));
+    GF_TRY(S.ptot.alloc((size_t)2 * B));
+    GF_TRY(S.pdis.alloc((size_t)2 * B));
     S.Bmax = B;
     return GF_OK;
 }
 
-/* forward pass of the whole ensemble on S.x (B rows) */
-static int forward(gf_sg_model &S, int B) {
+/* forward pass of the whole ensemble on S.x (B rows); keep_z: also store
+ * the hidden layers' pre-activations Z (the backward pass reads them) */
+static int forward(gf_sg_model &S, int B, bool keep_z) {
+    keep_z = keep_z || B < 4096 || B % GT != 0;   /* k_gemm always stores Z */
     const Layout &L = S.lay;
     const int64_t sW = (int64_t)L.total;
     const int64_t sAct = (int64_t)NHID * B * HID;
@@ -349,7 +497,7 @@ static int forward(gf_sg_model &S, int B) {
         a.bias = S.w.p + L.b[l]; a.sBias = sW;
         if (l < NHID) {
             a.epi = EPI_HIDDEN;
-            a.Cf = S.Z.p + (int64_t)l * B * HID; a.sCf = sAct; a.ldc = HID;
+            a.Cf = keep_z ? S.Z.p + (int64_t)l * B * HID : nullptr; a.sCf = sAct; a.ldc = HID;
             a.Cb = S.H.p + (int64_t)l * B * HID; a.sCb = sAct;
         } else {
             a.epi = EPI_OUTPUT;
@@ -366,7 +514,7 @@ static int train_step(gf_sg_model &S, int B, float lr, float w_tot, bool update)
     const int E = S.E;
     const int64_t sW = (int64_t)L.total;
     const int64_t sAct = (int64_t)NHID * B * HID;
-    GF_TRY(forward(S, B));
+    GF_TRY(forward(S, B, true));
     hipLaunchKernelGGL(k_loss, dim3((B + 127) / 128, E), dim3(128), 0, S.st, S.logits.p, (int64_t)B * OUT_PAD,
                        S.y.p, S.tot.p, B, w_tot, S.dlog.p, (int64_t)B * OUT_PAD, S.loss.p);
     GF_HIP(hipGetLastError());
@@ -560,28 +708,53 @@ int32_t gf_sg_train(gf_sg_model *S, const float *x, const float *y, const float 
 }
 
 /* ensemble prediction for n inputs: mean normalised SFS (bins 1..99),
- * mean log-total and the ensemble disagreement per input */
+ * mean log-total and the ensemble disagreement per input.  Batches of up
+ * to 65,536 inputs alternate between two output buffers: while batch i is
+ * computed on the model's stream, batch i - 1 is read back on the copy
+ * stream (each waits for the other through events). */
 int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, float *tot, float *dis) {
     if (!S || !x || n <= 0 || !sfs) { set_error("gf_sg_predict: invalid arguments"); return GF_ERR_ARG; }
     StreamScope scope(S->st);   /* buffers (re)allocated here belong to the model's stream */
-    const int B = std::min(65536, ((n + TILE - 1) / TILE) * TILE);
+    GF_HIP(S->cst.create());
+    const int B = std::min(65536, ((n + GT - 1) / GT) * GT);
     GF_TRY(ensure_batch(*S, B));
-    std::vector<float> xb((size_t)B * 6, 0.0f);
-    for (int off = 0; off < n; off += B) {
-        const int cnt = std::min(B, n - off);
-        memcpy(xb.data(), x + (size_t)off * 6, (size_t)cnt * 6 * sizeof(float));
-        if (cnt < B) memset(xb.data() + (size_t)cnt * 6, 0, (size_t)(B - cnt) * 6 * sizeof(float));
-        GF_TRY(load_batch(*S, xb.data(), nullptr, nullptr, B));
-        GF_TRY(forward(*S, B));
-        hipLaunchKernelGGL(k_predict_stats, dim3((B + 63) / 64), dim3(64), 0, S->st, S->logits.p, (int64_t)B * OUT_PAD,
-                           S->E, B, S->psfs.p, S->ptot.p, S->pdis.p);
-        GF_HIP(hipGetLastError());
-        /* the spectra (99 floats per input) through the pinned staging path */
-        GF_TRY(readback(sfs + (size_t)off * NBINS, S->psfs.p, (size_t)cnt * NBINS * sizeof(float), S->st));
-        if (tot) GF_HIP(hipMemcpyAsync(tot + off, S->ptot.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, S->st));
-        if (dis) GF_HIP(hipMemcpyAsync(dis + off, S->pdis.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, S->st));
-        GF_HIP(hipStreamSynchronize(S->st));
+    struct Events {   /* [0..1] computed, [2..3] read back, per output buffer */
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events() { for (auto v : e) if (v) (void)hipEventDestroy(v); }
+    } ev;
+    for (auto &v : ev.e) GF_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+    std::vector<float> xb[2];   /* a batch's inputs stay untouched until its compute has started */
+    for (auto &v : xb) v.assign((size_t)B * 6, 0.0f);
+    const int nb = (n + B - 1) / B;
+    for (int i = 0; i <= nb; i++) {
+        if (i < nb) {
+            const int off = i * B, cnt = std::min(B, n - off), u = i & 1;
+            memcpy(xb[u].data(), x + (size_t)off * 6, (size_t)cnt * 6 * sizeof(float));
+            if (cnt < B) memset(xb[u].data() + (size_t)cnt * 6, 0, (size_t)(B - cnt) * 6 * sizeof(float));
+            if (i >= 2) GF_HIP(hipStreamWaitEvent(S->st, ev.e[2 + u], 0));   /* buffer u read back */
+            GF_TRY(load_batch(*S, xb[u].data(), nullptr, nullptr, B));
+            GF_TRY(forward(*S, B, false));
+            hipLaunchKernelGGL(k_predict_stats, dim3((B + 3) / 4), dim3(256), 0, S->st, S->logits.p,
+                               (int64_t)B * OUT_PAD, S->E, B, S->psfs.p + (size_t)u * B * NBINS,
+                               S->ptot.p + (size_t)u * B, S->pdis.p + (size_t)u * B);
+            GF_HIP(hipGetLastError());
+            GF_HIP(hipEventRecord(ev.e[u], S->st));
+        }
+        if (i >= 1) {
+            const int j = i - 1, off = j * B, cnt = std::min(B, n - off), u = j & 1;
+            GF_HIP(hipStreamWaitEvent(S->cst, ev.e[u], 0));
+            /* the spectra (99 floats per input) through the pinned staging path */
+            GF_TRY(readback(sfs + (size_t)off * NBINS, S->psfs.p + (size_t)u * B * NBINS,
+                            (size_t)cnt * NBINS * sizeof(float), S->cst));
+            if (tot) GF_HIP(hipMemcpyAsync(tot + off, S->ptot.p + (size_t)u * B, (size_t)cnt * sizeof(float),
+                                           hipMemcpyDeviceToHost, S->cst));
+            if (dis) GF_HIP(hipMemcpyAsync(dis + off, S->pdis.p + (size_t)u * B, (size_t)cnt * sizeof(float),
+                                           hipMemcpyDeviceToHost, S->cst));
+            GF_HIP(hipEventRecord(ev.e[2 + u], S->cst));
+        }
     }
+    GF_HIP(hipStreamSynchronize(S->cst));
+    GF_HIP(hipStreamSynchronize(S->st));
     return GF_OK;
 }
 
@@ -591,7 +764,7 @@ int32_t gf_sg_forward(gf_sg_model *S, const float *x, int32_t n, float *logits) 
     StreamScope scope(S->st);   /* buffers (re)allocated here belong to the model's stream */
     GF_TRY(ensure_batch(*S, n));
     GF_TRY(load_batch(*S, x, nullptr, nullptr, n));
-    GF_TRY(forward(*S, n));
+    GF_TRY(forward(*S, n, false));
     GF_HIP(hipStreamSynchronize(S->st));
     for (int e = 0; e < S->E; e++)
         GF_HIP(hipMemcpy(logits + (size_t)e * n * OUT_PAD, S->logits.p + (size_t)e * n * OUT_PAD, (size_t)n * OUT_PAD * sizeof(float), hipMemcpyDeviceToHost));

@@ -122,6 +122,36 @@ def test_predict_is_the_ensemble_mean_of_forward():
     net.close()
 
 
+def test_large_batch_forward_kernel_gives_the_same_bits():
+    """Batches of 4,096 rows and more run the forward products through the
+    128 x 128-tile kernel (double-buffered LDS, XCD-aware tile order, no
+    pre-activation store); each output accumulates its k-steps in the same
+    order as the 64 x 64-tile kernel that smaller batches use, so the logits
+    of a 4,096-row batch equal those of the same rows sent 64 at a time, bit
+    for bit, for every ensemble member."""
+    net = SG.Network(ensemble=2, seed=21)
+    x, _, _ = inputs(4096, 9)
+    big = net.forward(x)
+    small = np.concatenate([net.forward(x[i:i + 64]) for i in range(0, 4096, 64)], axis=1)
+    assert np.array_equal(big.view(np.uint32), small.view(np.uint32))
+    net.close()
+
+
+def test_predict_pipelined_batches_match_one_batch():
+    """Prediction over 2^17 + 100 inputs runs three batches (65,536 rows at
+    most) through two output buffers, the read-back of one batch beside the
+    compute of the next: the rows of each batch equal, bit for bit, a
+    prediction of that batch's inputs alone."""
+    net = SG.Network(ensemble=4, seed=22)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, ((1 << 17) + 100, 6)).astype(np.float32)
+    sfs, tot, dis = net.predict(x)
+    for lo, hi in [(0, 65536), (65536, 131072), (131072, len(x))]:
+        s1, t1, d1 = net.predict(x[lo:hi])
+        assert np.array_equal(sfs[lo:hi], s1) and np.array_equal(tot[lo:hi], t1) and np.array_equal(dis[lo:hi], d1), lo
+    net.close()
+
+
 def test_invalid_arguments_are_rejected():
     net = SG.Network(ensemble=1, seed=1)
     x, y, t = inputs(100, 6)      # not a multiple of 64
