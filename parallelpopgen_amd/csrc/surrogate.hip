@@ -174,8 +174,10 @@ constexpr int GRS = 256 / GR;  /* rows one pass of the workgroup copies */
 constexpr int GP = GT / GRS;   /* passes per operand and stage */
 
 __global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs g, int MT, int NT, int T) {
-    __shared__ bf16 As[2][GT][GLDS];
-    __shared__ bf16 Bs[2][GT][GLDS];
+    /* operand stages; after the K loop the same LDS stages the output tile */
+    __shared__ __attribute__((aligned(16))) bf16 sm[2][2][GT][GLDS];
+    auto &As = sm[0];
+    auto &Bs = sm[1];
     const int b = (int)blockIdx.x;
     const int xcd = b & 7, r = b >> 3;
     const int per = T >> 3, rem = T & 7;
@@ -244,25 +246,73 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs g, int MT, int N
         }
         __syncthreads();
     }
-    /* epilogue: accumulator r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32] of its block */
+    /* epilogue: accumulator r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32]
+     * of its block.  The tile goes through LDS (free after the loop's last
+     * barrier) so that global stores are whole 16-byte row pieces: a lane's
+     * accumulators run down a column.  Row strides of 144 bf16 / 136 fp32
+     * put the two half-waves (rows 4 apart) on different LDS banks. */
+    if (g.epi == EPI_HIDDEN && g.Cf) {   /* pre-activations kept as well: direct stores */
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int n = n0 + wn * 64 + j * 32 + fr;
-        const float bias = g.bias ? g.bias[z * g.sBias + n] : 0.0f;
+        for (int j = 0; j < 2; j++) {
+            const int n = n0 + wn * 64 + j * 32 + fr;
+            const float bias = g.bias[z * g.sBias + n];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int m = m0 + wm * 64 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                const int64_t o = (int64_t)m * g.ldc + n;
-                const float v = acc[i][j][q] + bias;
-                if (g.epi == EPI_HIDDEN) {
-                    if (g.Cf) g.Cf[z * g.sCf + o] = v;
-                    g.Cb[z * g.sCb + o] = (bf16)gelu(v);
-                } else {
+                for (int q = 0; q < 16; q++) {
+                    const int m = m0 + wm * 64 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    const int64_t o = (int64_t)m * g.ldc + n;
+                    const float v = acc[i][j][q] + bias;
                     g.Cf[z * g.sCf + o] = v;
+                    g.Cb[z * g.sCb + o] = (bf16)gelu(v);
                 }
-            }
+        }
+        return;
+    }
+    if (g.epi == EPI_HIDDEN) {
+        constexpr int S = GT + 16;
+        static_assert(GT * S * sizeof(bf16) <= sizeof(sm), "bf16 output tile fits the operand LDS");
+        bf16 *cs = &sm[0][0][0][0];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int c = wn * 64 + j * 32 + fr;
+            const float bias = g.bias[z * g.sBias + n0 + c];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const int rr = wm * 64 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    cs[rr * S + c] = (bf16)gelu(acc[i][j][q] + bias);
+                }
+        }
+        __syncthreads();
+        bf16 *dst = g.Cb + z * g.sCb;
+        for (int p = t; p < GT * (GT / 8); p += 256) {
+            const int rr = p / (GT / 8), c = (p % (GT / 8)) * 8;
+            *(bf16x8 *)&dst[(int64_t)(m0 + rr) * g.ldc + n0 + c] = *(const bf16x8 *)&cs[rr * S + c];
+        }
+    } else {
+        constexpr int S = GT + 8;
+        static_assert(GT * S * sizeof(float) <= sizeof(sm), "fp32 output tile fits the operand LDS");
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        float *cs = reinterpret_cast<float *>(&sm[0][0][0][0]);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int c = wn * 64 + j * 32 + fr;
+            const float bias = g.bias ? g.bias[z * g.sBias + n0 + c] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const int rr = wm * 64 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    cs[rr * S + c] = acc[i][j][q] + bias;
+                }
+        }
+        __syncthreads();
+        float *dst = g.Cf + z * g.sCf;
+        for (int p = t; p < GT * (GT / 4); p += 256) {
+            const int rr = p / (GT / 4), c = (p % (GT / 4)) * 4;
+            *(f32x4 *)&dst[(int64_t)(m0 + rr) * g.ldc + n0 + c] = *(const f32x4 *)&cs[rr * S + c];
         }
     }
 }
