@@ -102,23 +102,35 @@ struct StreamScope {
  * at once; a block of unknown stream waits for the device at release.  An
  * allocation takes the smallest cached block of its size class up to twice
  * its size, preferring blocks whose event has completed.  At most kKeep
- * bytes are kept per device (the oldest blocks are freed first); blocks
- * above kMaxBlock are freed at once; a failed hipMalloc empties the
+ * bytes (32 GiB, a ninth of an MI355X's memory) are kept per device (the
+ * oldest blocks are freed first); blocks above kMaxBlock are freed at once; a failed hipMalloc empties the
  * device's cache and tries again; gf_release_device_cache empties it on
  * request.  The cache lives until the process ends. */
 class DevCache {
 public:
-    static constexpr size_t kKeep = (size_t)16 << 30;
-    static constexpr size_t kMaxBlock = (size_t)4 << 30;
+    static constexpr size_t kKeep = (size_t)32 << 30;
+    static constexpr size_t kMaxBlock = (size_t)16 << 30;
 
     static DevCache &get() {
         static DevCache *c = new DevCache();   /* never destroyed: no teardown-order hazard */
         return *c;
     }
 
-    /* size class: powers of two from 256 B below 1 MiB, 2 MiB steps above */
+    /* size class: powers of two from 256 B below 1 MiB, 2 MiB steps up to
+     * 64 MiB, then four classes per power of two (64, 80, 96, 112, 128,
+     * 160 ... MiB): a workload whose large buffers vary by some percent
+     * from call to call (batched runs of different draws) asks for the
+     * same few classes again, instead of leaving the cache full of blocks
+     * that are each a little too small */
     static size_t size_class(size_t bytes) {
-        if (bytes >= ((size_t)1 << 20)) return (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+        const size_t MiB = (size_t)1 << 20;
+        if (bytes >= 64 * MiB) {
+            size_t p = 64 * MiB;
+            while (2 * p <= bytes) p <<= 1;
+            const size_t q = p / 4;
+            return (bytes + q - 1) / q * q;
+        }
+        if (bytes >= MiB) return (bytes + 2 * MiB - 1) & ~(2 * MiB - 1);
         size_t c = 256;
         while (c < bytes) c <<= 1;
         return c;
