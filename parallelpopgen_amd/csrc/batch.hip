@@ -656,8 +656,11 @@ struct BatchRun {
             if (t > 2000000000LL / WG) { set_error("gf_batch_run: more than 2e9 slots"); return GF_ERR_UNSUPPORTED; }
         }
         GF_TRY(allocate_layout(t));
+        /* host arrays given to asynchronous copies live until the
+         * synchronisation at the end */
+        std::vector<int64_t> out_offs;
         if (ns > 0) {
-            std::vector<int64_t> out_offs(ns);
+            out_offs.resize(ns);
             for (int k = 0; k < ns; k++) {
                 const int s = seg_sim[k];
                 segs[k].slot_base = (int64_t)host[s].t0 * WG;
