@@ -172,8 +172,13 @@ __device__ __forceinline__ void generation_update(float *x, const Row &R, int g,
         const float mean = xs * N;
         const float var = (1.0f - xs) * mean;
         int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N, ncdf);
-        j = j < 0 ? 0 : (j > Nc ? Nc : j);
-        nx[p] = (float)((double)(float)j * R.invN[p]);   /* == (float)j / N */
+        j = min(max(j, 0), Nc);
+        /* (float)j / N, correctly rounded: up to N = 2^22 in three float
+         * operations with r = RN(1/N), the row's newfreq (quot_count),
+         * above through double (invN); the test is on the integer Nc, a
+         * scalar comparison */
+        const float fj = (float)j;
+        nx[p] = Nc <= 4194304 ? quot_count(fj, N, R.newfreq[p]) : (float)((double)fj * R.invN[p]);
     }
 #pragma unroll
     for (int p = 0; p < P; p++) x[p] = nx[p];

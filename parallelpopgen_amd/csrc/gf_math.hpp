@@ -178,13 +178,9 @@ struct alignas(16) NcdfSeg { float c0, c1, c2, c3; };
 
 GF_HD const NcdfSeg *ncdf_table() { return reinterpret_cast<const NcdfSeg *>(kNcdfSeg); }
 
-GF_HD float normcdfinv_tab(const NcdfSeg *tab, float p) {
+/* the table's function on its domain, 2^-33 <= p <= 1 - 2^-24 */
+GF_HD float normcdfinv_core(const NcdfSeg *tab, float p) {
     GF_STRICT_FP
-    /* the table's domain [2^-33, 1 - 2^-24] (u01's range, its top end
-     * clamped): anything below 2^-33, and NaN, is taken as 2^-33, so the
-     * segment index stays in the table for every input */
-    p = p > 0x1p-33f ? p : 0x1p-33f;
-    if (p > 0.99999994039535522f) p = 0.99999994039535522f;
     const float q = p - 0.5f;
     const float s = q < 0.0f ? p : 1.0f - p;        /* [2^-33, 1/2], exact */
     const uint32_t b = f2bits(s);
@@ -192,6 +188,16 @@ GF_HD float normcdfinv_tab(const NcdfSeg *tab, float p) {
     const float t = (float)(b & ((1u << GF_NCDF_T_BITS) - 1u));
     const float r = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.c3, t, c.c2), t, c.c1), t, c.c0);
     return q < 0.0f ? r : -r;
+}
+
+GF_HD float normcdfinv_tab(const NcdfSeg *tab, float p) {
+    GF_STRICT_FP
+    /* the table's domain [2^-33, 1 - 2^-24] (u01's range, its top end
+     * clamped): anything below 2^-33, and NaN, is taken as 2^-33, so the
+     * segment index stays in the table for every input */
+    p = p > 0x1p-33f ? p : 0x1p-33f;
+    if (p > 0.99999994039535522f) p = 0.99999994039535522f;
+    return normcdfinv_core(tab, p);
 }
 
 GF_HD float normcdfinv_f(float p) { return normcdfinv_tab(ncdf_table(), p); }
@@ -375,7 +381,10 @@ GF_HD int binom_draw(uint32_t bits, float mean, float var, float N, const NcdfSe
         if (m != 0.0f) k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv(u, m);
         return lo ? k : f2i_sat(N - (float)k);
     }
-    return round_away_nonneg(normcdfinv_tab(ncdf, u) * sqrt_pos(var) + mean);
+    /* u01 gives 2^-33 <= u <= 1 (never NaN): of normcdfinv_tab's clamps
+     * only the top one can act, and the result is the same */
+    return round_away_nonneg(normcdfinv_core(ncdf, u > 0.99999994039535522f ? 0.99999994039535522f : u) *
+                                 sqrt_pos(var) + mean);
 }
 
 /* approximate Poisson draw (reference shared.cuh:251-256) */
@@ -389,6 +398,21 @@ GF_HD int pois_draw(uint32_t bits, float mean, float var, float N) {
         return lo ? k : f2i_sat(N - (float)k);
     }
     return f2i_sat(round_away(normcdfinv_f(u) * sqrt_pos(var) + mean));
+}
+
+/* (float)j / N, correctly rounded, for integers 0 <= j <= N <= 2^22 given
+ * r = RN(1/N), in three float operations: q = j r is within 1.5 ulp of
+ * j/N; e = j - q N is exact (a multiple of ulp(q) below 3N ulp(q) <
+ * 2^24 ulp(q)); q + e r differs from j/N by at most |e| |r - 1/N| <
+ * 1.5 ulp 2^-24, less than the distance from j/N to the nearest rounding
+ * midpoint (at least ulp / 2N, and j/N with N <= 2^22 is never a
+ * midpoint), so the fma's one rounding gives RN(j/N).  Checked
+ * exhaustively for N <= 32768 and on sampled larger N by
+ * tests/cpp/quotient.hip. */
+GF_HD float quot_count(float fj, float N, float r) {
+    GF_STRICT_FP
+    const float q = fj * r;
+    return __builtin_fmaf(__builtin_fmaf(-q, N, fj), r, q);
 }
 
 /* ----- model helpers ----- */
