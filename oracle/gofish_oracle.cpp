@@ -22,6 +22,9 @@
  *   exp_f, log_f       error bound vs long double (numpy longdouble)
  *   normcdfinv_f       error bound vs scipy.special.ndtri over u01's range;
  *                      symmetry; monotonicity
+ *   expneg_tab         error bound vs long double exp over [0, 6] (every
+ *                      float there, tests/cpp/tables.hip); exp(-0) = 1;
+ *                      non-increasing to within one ulp
  *   poiscdfinv/binomcdfinv  brute-force long-double CDF inversion with the
  *                      same 70-term truncation and mean thresholds
  *   binom_draw, pois_draw   first two moments over many draws
@@ -62,6 +65,9 @@
  *     library) is a segmented cubic table over the float bits of
  *     min(u, 1 - u), fitted by parallelpopgen_amd/csrc/fit_normcdfinv.py;
  *     u is clamped to at most 1 - 2^-24 (u == 1.0f would give +inf);
+ *   - the Poisson inversion's first term exp(-mean) (CUDA's expf in the
+ *     reference) is, for 0 <= mean <= 6, a segmented cubic table in mean,
+ *     fitted by parallelpopgen_amd/csrc/fit_expneg.py;
  *   - a population that is not drifted in a generation (not yet arisen,
  *     extinct) gets frequency 0 instead of NaN/stale memory;
  *   - the MSE integral uses a fixed-order two-level float scan
@@ -257,11 +263,38 @@ static const float recip[69] = {
     0x1.dae6080000000p-7f,
 };
 
+/* exp(-m) for 0 <= m <= 6: the segment table of the library
+ * (gf_expneg_table.h, fitted by parallelpopgen_amd/csrc/fit_expneg.py):
+ * segment k = min(floor(16 m), 95), cubic in t = m - k/16 by Horner's rule
+ * with fused multiply-adds.  Pinned against exp in long double. */
+#include "../parallelpopgen_amd/csrc/gf_expneg_table.h"
+
+static float expneg_tab(float m) {
+    int k = (int)(m * 16.0f);
+    k = k < GF_EXPNEG_SEGMENTS - 1 ? k : GF_EXPNEG_SEGMENTS - 1;
+    k = k > 0 ? k : 0;
+    const float t = fmaf((float)k, -0.0625f, m);
+    const float *c = kExpNegSeg + 4 * k;
+    return fmaf(fmaf(fmaf(c[3], t, c[2]), t, c[1]), t, c[0]);
+}
+
 /* Poisson CDF inversion, reference shared.cuh:168-249.  The recurrence
  * multiplies by fl(1/j) where the reference divides by j (its
- * --use_fast_math build evaluates that quotient as x * rcp(j) as well). */
+ * --use_fast_math build evaluates that quotient as x * rcp(j) as well).
+ * The first term exp(-mean) is expneg_tab for 0 <= mean <= 6 (the draws'
+ * Poisson tails), exp_f beyond. */
+static int poiscdfinv_from(float r, float mean, float emu);
+
 static int poiscdfinv(float r, float mean) {
-    float emu = exp_f(-1.0f * mean);
+    return poiscdfinv_from(r, mean, (mean >= 0.0f && mean <= 6.0f) ? expneg_tab(mean) : exp_f(-1.0f * mean));
+}
+
+/* the approximate binomial's Poisson tails (mean <= 6, or N - mean <= 6):
+ * the table whatever the sign (a negative mean needs h outside [0, 1]) */
+static int poiscdfinv_tail(float r, float mean) { return poiscdfinv_from(r, mean, expneg_tab(mean)); }
+
+/* the inversion given its first term emu = exp(-mean) */
+static int poiscdfinv_from(float r, float mean, float emu) {
     float cdf = emu;
     if (cdf >= r) return 0;
     for (int j = 1; j < 70; j++) {
@@ -297,13 +330,13 @@ static int binom_draw(uint32_t rbits, float mean, float var, float N) {
     float u = u01(rbits);
     if (mean <= MEAN_BOUNDARY_NORM) {
         if (N < N_BOUNDARY_POIS_BINOM) return binomcdfinv(u, mean, mean / N, N);
-        return poiscdfinv(u, mean);
+        return poiscdfinv_tail(u, mean);
     } else if (mean >= N - MEAN_BOUNDARY_NORM) {
         if (N < N_BOUNDARY_POIS_BINOM) {
             float nm = N - mean;
             return f2i_sat(N - (float)binomcdfinv(u, nm, nm / N, N));
         }
-        return f2i_sat(N - (float)poiscdfinv(u, N - mean));
+        return f2i_sat(N - (float)poiscdfinv_tail(u, N - mean));
     }
 #if ORC_PLANT == 15
     return f2i_sat(round_away(normcdfinv_f(u) * var + mean));   /* planted: variance for sd */
@@ -939,6 +972,7 @@ void orc_u01_array(const uint32_t *in, float *out, int64_t n) { for (int64_t i =
 void orc_exp_array(const float *in, float *out, int64_t n) { for (int64_t i = 0; i < n; i++) out[i] = exp_f(in[i]); }
 void orc_log_array(const float *in, float *out, int64_t n) { for (int64_t i = 0; i < n; i++) out[i] = log_f(in[i]); }
 void orc_normcdfinv_array(const float *in, float *out, int64_t n) { for (int64_t i = 0; i < n; i++) out[i] = normcdfinv_f(in[i]); }
+void orc_expneg_array(const float *in, float *out, int64_t n) { for (int64_t i = 0; i < n; i++) out[i] = expneg_tab(in[i]); }
 void orc_poiscdfinv_array(const float *r, const float *mean, int32_t *out, int64_t n) {
     for (int64_t i = 0; i < n; i++) out[i] = poiscdfinv(r[i], mean[i]);
 }

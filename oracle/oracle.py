@@ -39,6 +39,7 @@ _SIGS = {
     "orc_exp_array": (None, [_VP, _VP, _I64]),
     "orc_log_array": (None, [_VP, _VP, _I64]),
     "orc_normcdfinv_array": (None, [_VP, _VP, _I64]),
+    "orc_expneg_array": (None, [_VP, _VP, _I64]),
     "orc_poiscdfinv_array": (None, [_VP, _VP, _VP, _I64]),
     "orc_binomcdfinv_array": (None, [_VP, _VP, _VP, _VP, _VP, _I64]),
     "orc_binom_draw_array": (None, [_VP, _VP, _VP, _VP, _VP, _I64]),
@@ -126,6 +127,11 @@ def log_f(x):
 
 def normcdfinv(x):
     return _unary("orc_normcdfinv_array", x)
+
+
+def expneg(x):
+    """exp(-x) for 0 <= x <= 6 from the library's segment table"""
+    return _unary("orc_expneg_array", x)
 
 
 def round_away(x):

@@ -169,7 +169,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     __shared__ unsigned int sh_wcount[WG / 64];
     __shared__ int sh_pre[WG + 1];
     __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
-    load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
+    __shared__ ExpSeg sh_eneg[GF_EXPNEG_SEGMENTS];
+    load_draw_tables(sh_ncdf, sh_eneg);   /* published by the tile loop's first barrier */
     uint64_t live = 0;          /* this thread's slots' live mutation-generations */
     for (int gt = blockIdx.x;; gt += gridDim.x) {
         __syncthreads();
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             const int g = S.g + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
-                generation_update<1>(&x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf);
+                generation_update<1>(&x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf, sh_eneg);
             }
             keep = valid && born <= t && keep_rule<1>(&x, R.present_mask, false);
             /* one population: kept over one run of generations (lost and

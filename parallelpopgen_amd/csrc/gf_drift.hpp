@@ -135,12 +135,12 @@ struct FunctorSel {
  * `refresh` is set (at a block boundary, or in the first generation a slot
  * is drifted in this launch), also for a population that is empty in this
  * generation, so that it is current if the population is founded later in
- * the same block.  ncdf is the workgroup's LDS copy of the inverse normal's
- * segment table (load_ncdf_table). */
+ * the same block.  ncdf and etab are the workgroup's LDS copies of the
+ * inverse normal's and exp(-m)'s segment tables (load_draw_tables). */
 template <int P, typename Row, typename Sel>
 __device__ __forceinline__ void generation_update(float *x, const Row &R, int g, bool refresh, u32x4 *rc,
                                                   uint32_t k0, uint32_t k1, uint32_t tid, int ogen, int opop,
-                                                  const Sel &sel, const NcdfSeg *ncdf) {
+                                                  const Sel &sel, const NcdfSeg *ncdf, const ExpSeg *etab) {
     GF_STRICT_FP
     float nx[P];
 #pragma unroll
@@ -171,7 +171,7 @@ __device__ __forceinline__ void generation_update(float *x, const Row &R, int g,
         const float N = R.fN[p];
         const float mean = xs * N;
         const float var = (1.0f - xs) * mean;
-        int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N, ncdf);
+        int j = binom_draw(lane_of(rc[p], g & 3), mean, var, N, ncdf, etab);
         j = min(max(j, 0), Nc);
         /* (float)j / N, correctly rounded: up to N = 2^22 in three float
          * operations with r = RN(1/N), the row's newfreq (quot_count),
@@ -184,11 +184,13 @@ __device__ __forceinline__ void generation_update(float *x, const Row &R, int g,
     for (int p = 0; p < P; p++) x[p] = nx[p];
 }
 
-/* copies the inverse normal's segment table into LDS (all threads of the
- * workgroup; the caller's next barrier publishes it) */
-__device__ __forceinline__ void load_ncdf_table(NcdfSeg *sh) {
+/* copies the inverse normal's and exp(-m)'s segment tables into LDS (all
+ * threads of the workgroup; the caller's next barrier publishes them) */
+__device__ __forceinline__ void load_draw_tables(NcdfSeg *sh, ExpSeg *she) {
     const NcdfSeg *tab = ncdf_table();
     for (int k = threadIdx.x; k < GF_NCDF_SEGMENTS; k += WG) sh[k] = tab[k];
+    const ExpSeg *etab = expneg_table();
+    for (int k = threadIdx.x; k < GF_EXPNEG_SEGMENTS; k += WG) she[k] = etab[k];
 }
 
 /* locate a slot born inside the block: generation index t and population */
@@ -239,7 +241,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     __shared__ int sh_tile;
     __shared__ unsigned int sh_wc[WG / 64];
     __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
-    load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
+    __shared__ ExpSeg sh_eneg[GF_EXPNEG_SEGMENTS];
+    load_draw_tables(sh_ncdf, sh_eneg);   /* published by the tile loop's first barrier */
     uint64_t cnt = 0;        /* one population: lane t counts generation t */
     uint64_t wave_cnt = 0;   /* batches: the wave's live mutation-generations */
     /* tiles are handed out dynamically (an atomic ticket): their cost varies
@@ -298,7 +301,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             const int g = S.g0 + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
-                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf);
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf, sh_eneg);
             }
             keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             const uint64_t b = __ballot(keep);
@@ -422,7 +425,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
     __shared__ unsigned long long sh_live[64];
     __shared__ int sh_diff[KMAX + 1];
     __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
-    load_ncdf_table(sh_ncdf);   /* published by the tile loop's first barrier */
+    __shared__ ExpSeg sh_eneg[GF_EXPNEG_SEGMENTS];
+    load_draw_tables(sh_ncdf, sh_eneg);   /* published by the tile loop's first barrier */
     if (threadIdx.x < 64) sh_live[threadIdx.x] = 0;
     if (threadIdx.x <= KMAX) sh_diff[threadIdx.x] = 0;
     const int M0 = *F.M_in;
@@ -495,7 +499,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_
             const int g = S.g0 + 1 + t;
             if (valid && born < t) {
                 const bool refresh = ((g & 3) == 0) || (t == born + 1);
-                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel, sh_ncdf);
+                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, sel, sh_ncdf, sh_eneg);
             }
             keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
             if constexpr (P == 1) {

@@ -296,10 +296,36 @@ GF_HD float exp_f_mid(float x) {
     return p * pow2i((int)n);
 }
 
+/* exp(-m) for 0 <= m <= 6 (the first term of the Poisson inversion), from
+ * a segment table (gf_expneg_table.h, fitted by parallelpopgen_amd/csrc/
+ * fit_expneg.py): segment k = min(floor(16 m), 95), a cubic in t =
+ * m - k/16 (exact), Horner's rule with fused multiply-adds; exp(-0) = 1.
+ * Error against exp at most 1.5 ulp over every float in [0, 6]
+ * (tests/cpp/tables.hip).  9 VALU operations against exp_f_mid's 17.
+ * `tab` is the table itself, or its LDS copy in the drift kernels. */
+#include "gf_expneg_table.h"
+
+struct alignas(16) ExpSeg { float c0, c1, c2, c3; };
+
+GF_HD const ExpSeg *expneg_table() { return reinterpret_cast<const ExpSeg *>(kExpNegSeg); }
+
+GF_HD float expneg_tab(const ExpSeg *tab, float m) {
+    GF_STRICT_FP
+    int k = (int)(m * 16.0f);   /* m >= 0: truncation is floor */
+    /* kept in the table for any input (a negative m, possible only with
+     * h outside [0, 1], extrapolates segment 0) */
+    k = k < GF_EXPNEG_SEGMENTS - 1 ? k : GF_EXPNEG_SEGMENTS - 1;
+    k = k > 0 ? k : 0;
+    const float t = __builtin_fmaf((float)k, -0.0625f, m);
+    const ExpSeg c = tab[k];
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.c3, t, c.c2), t, c.c1), t, c.c0);
+}
+
 /* Poisson CDF inversion truncated as in the reference (shared.cuh:168-249):
  * the first j with CDF(j) >= r, or the early-exit index of `mean` if that
- * comes first, or 70.  Callers pass 0 <= mean <= 6 (the Poisson tails of
- * the approximate binomial), where exp_f_mid(-mean) == exp_f(-mean).  The
+ * comes first, or 70.  The first term exp(-mean) comes from the table
+ * for 0 <= mean <= 6 (the draws' Poisson tails), else from exp_f_mid
+ * (equal to exp_f for the means callers pass, up to 87).  The
  * loop stays rolled (lanes leave it as they finish) and takes four
  * reciprocals per trip in one scalar load.  A trip computes its four terms
  * straight through; the partial sums d0 <= d1 <= d2 <= d3 are
@@ -307,9 +333,23 @@ GF_HD float exp_f_mid(float x) {
  * d3 >= r or the early exit falls inside it, and then the answer is the
  * number of partial sums below r, capped at the early exit.  The
  * recurrence and the comparisons are those of one term at a time. */
-GF_HD int poiscdfinv(float r, float mean) {
+GF_HD int poiscdfinv_from(float r, float mean, float emu);
+
+GF_HD int poiscdfinv(float r, float mean, const ExpSeg *etab = expneg_table()) {
     GF_STRICT_FP
-    float emu = exp_f_mid(-1.0f * mean);
+    return poiscdfinv_from(r, mean, (mean >= 0.0f && mean <= 6.0f) ? expneg_tab(etab, mean) : exp_f_mid(-1.0f * mean));
+}
+
+/* the draws' Poisson tails: 0 < mean <= 6, so always the table (the same
+ * value as poiscdfinv) */
+GF_HD int poiscdfinv_tail(float r, float mean, const ExpSeg *etab) {
+    GF_STRICT_FP
+    return poiscdfinv_from(r, mean, expneg_tab(etab, mean));
+}
+
+/* the inversion given its first term emu = exp(-mean) */
+GF_HD int poiscdfinv_from(float r, float mean, float emu) {
+    GF_STRICT_FP
     float cdf = emu;
     if (cdf >= r) return 0;
     /* exit_index for mean <= 14 without its branches (callers pass <= 6) */
@@ -370,7 +410,8 @@ GF_HD int round_away_nonneg(float v) {
  * the inversion.  The result equals the reference's draw wherever that is
  * >= 0; where the normal approximation goes below 0 it is some value <= 0
  * (round_away_nonneg), and the caller clamps the result to [0, N]. */
-GF_HD int binom_draw(uint32_t bits, float mean, float var, float N, const NcdfSeg *ncdf = ncdf_table()) {
+GF_HD int binom_draw(uint32_t bits, float mean, float var, float N, const NcdfSeg *ncdf = ncdf_table(),
+                     const ExpSeg *etab = expneg_table()) {
     GF_STRICT_FP
     const float u = u01(bits);
     const bool lo = mean <= 6.0f;
@@ -378,7 +419,7 @@ GF_HD int binom_draw(uint32_t bits, float mean, float var, float N, const NcdfSe
     if (lo || hi) {
         const float m = lo ? mean : N - mean;
         int k = 0;
-        if (m != 0.0f) k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv(u, m);
+        if (m != 0.0f) k = (N < 100.0f) ? binomcdfinv(u, m, m / N, N) : poiscdfinv_tail(u, m, etab);
         return lo ? k : f2i_sat(N - (float)k);
     }
     /* u01 gives 2^-33 <= u <= 1 (never NaN): of normcdfinv_tab's clamps

@@ -90,6 +90,22 @@ def _u01_values(bits):
     return (b * np.float32(2.3283064365386963e-10) + np.float32(1.1641532182693481e-10)).astype(np.float32)
 
 
+def test_expneg_table_error_bound(oracle):
+    """the oracle's exp(-m) table (shared with the library) against long
+    double exp: within 1.5 ulp on random m in [0, 6], at the segment knots
+    and just below them, and at tiny m; exp(-0) = 1 exactly"""
+    rng = np.random.default_rng(12)
+    knots = np.arange(97) / 16.0
+    m = np.concatenate([rng.uniform(0, 6, 300000), knots, np.nextafter(knots[1:], 0),
+                        2.0 ** -np.arange(1, 60)]).astype(np.float32)
+    got = oracle.expneg(m).astype(np.longdouble)
+    want = np.exp(-m.astype(np.longdouble))
+    ulp = np.spacing(want.astype(np.float32)).astype(np.longdouble)
+    err = np.abs(got - want) / ulp
+    assert err.max() <= 1.5, (float(err.max()), float(m[np.argmax(err)]))
+    assert oracle.expneg(np.zeros(1, np.float32))[0] == 1.0
+
+
 def test_normcdfinv_error_bound(oracle):
     # Domain: every value u01 can return, [2^-33, 1].  The segment table is
     # fitted to float accuracy (1.5 ulp where |z| >= 0.1, 4.8e-7 absolute
