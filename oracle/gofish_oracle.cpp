@@ -270,7 +270,7 @@ static const float recip[69] = {
 #include "../parallelpopgen_amd/csrc/gf_expneg_table.h"
 
 static float expneg_tab(float m) {
-    int k = (int)(m * 16.0f);
+    int k = f2i_sat(m * 16.0f);
     k = k < GF_EXPNEG_SEGMENTS - 1 ? k : GF_EXPNEG_SEGMENTS - 1;
     k = k > 0 ? k : 0;
     const float t = fmaf((float)k, -0.0625f, m);

@@ -311,7 +311,7 @@ GF_HD const ExpSeg *expneg_table() { return reinterpret_cast<const ExpSeg *>(kEx
 
 GF_HD float expneg_tab(const ExpSeg *tab, float m) {
     GF_STRICT_FP
-    int k = (int)(m * 16.0f);   /* m >= 0: truncation is floor */
+    int k = f2i_sat(m * 16.0f);   /* m >= 0: truncation is floor */
     /* kept in the table for any input (a negative m, possible only with
      * h outside [0, 1], extrapolates segment 0) */
     k = k < GF_EXPNEG_SEGMENTS - 1 ? k : GF_EXPNEG_SEGMENTS - 1;
