@@ -79,16 +79,6 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *as_const(c
     return (const __attribute__((address_space(4))) T *)p;
 }
 
-/* a segment descriptor read with scalar loads (see as_const) */
-template <int P>
-__device__ __forceinline__ SegLaunch<P> load_seg(const SegLaunch<P> *p) {
-    const auto &c = *as_const(p);
-    SegLaunch<P> S;
-    S.rows = c.rows; S.live = c.live; S.base = c.base; S.M0 = c.M0; S.Mend = c.Mend; S.cum_base = c.cum_base;
-    S.K = c.K; S.g0 = c.g0; S.t0 = c.t0; S.k0 = c.k0; S.k1 = c.k1; S.pad = c.pad;
-    return S;
-}
-
 /* keep rule of the compaction flag (reference go_fish_impl.cuh:153-166) */
 template <int P>
 __device__ __forceinline__ bool keep_rule(const float *x, int present_mask, bool preserved) {
@@ -218,135 +208,10 @@ __device__ __forceinline__ int seg_of_tile(const int *__restrict__ tile_first, i
     return lo;
 }
 
-/* Both drift kernels ask for 8 waves per SIMD (amdgpu_waves_per_eu): the
+/* The drift kernels ask for 8 waves per SIMD (amdgpu_waves_per_eu): the
  * compiler then keeps them within 96 SGPRs (spilling a few to VGPR lanes),
- * which measured 5 % faster than 7 waves with no spills.
- *
- * Advance every slot of every segment by its K generations.
- * BATCH=false: one segment, per-generation live counts (lane t of each wave
- * accumulates generation t, KMAX <= 64).  BATCH=true: tile_seg maps tiles to
- * segments and only the total of live mutation-generations is counted, in
- * seg0.live[0..63] (summed by the host). */
-template <int P, bool BATCH>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_advance(float *__restrict__ freq, int64_t pitch,
-                                                gf_mutid *__restrict__ ids,
-                                                const SegLaunch<P> seg0,
-                                                const SegLaunch<P> *__restrict__ segs,
-                                                const int *__restrict__ tile_first, int nseg,
-                                                int num_tiles, unsigned int *__restrict__ ticket,
-                                                unsigned long long *__restrict__ masks,
-                                                unsigned int *__restrict__ tile_counts) {
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    __shared__ int sh_tile;
-    __shared__ unsigned int sh_wc[WG / 64];
-    __shared__ NcdfSeg sh_ncdf[GF_NCDF_SEGMENTS];
-    __shared__ ExpSeg sh_eneg[GF_EXPNEG_SEGMENTS];
-    load_draw_tables(sh_ncdf, sh_eneg);   /* published by the tile loop's first barrier */
-    uint64_t cnt = 0;        /* one population: lane t counts generation t */
-    uint64_t wave_cnt = 0;   /* batches: the wave's live mutation-generations */
-    /* tiles are handed out dynamically (an atomic ticket): their cost varies
-     * with the simulation they belong to */
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) sh_tile = (int)atomicAdd(ticket, 1u);
-        __syncthreads();
-        const int tile = sh_tile;
-        if (tile >= num_tiles) break;
-        const int s = BATCH ? seg_of_tile(tile_first, nseg, tile) : 0;
-        SegLaunch<P> S = seg0;
-        if (BATCH) S = load_seg<P>(segs + s);
-        const int li = (BATCH ? (tile - tile_first[s]) : tile) * WG + (int)threadIdx.x;
-        const bool valid = li < S.Mend;
-        const int64_t slot = S.base + li;
-        const GenRow<P> *rows = S.rows;
-        float x[P];
-        int ogen = 0, opop = 0, born = -1;
-        uint32_t tid = 0;
-        bool preserved = false;
-#pragma unroll
-        for (int p = 0; p < P; p++) x[p] = 0.0f;
-        if (valid) {
-            if (li < S.M0) {
-#pragma unroll
-                for (int p = 0; p < P; p++) x[p] = freq[p * pitch + slot];
-                const gf_mutid id = ids[slot];
-                ogen = id.origin_generation;
-                opop = id.origin_population;
-                preserved = id.origin_threadID < 0;
-                tid = (uint32_t)(preserved ? -id.origin_threadID : id.origin_threadID);
-            } else {
-                const int rel = li - S.M0 + S.cum_base;
-                int tb, pb;
-                find_birth<P>(rows, S.K, rel, tb, pb);
-                born = tb;
-                ogen = S.g0 + 1 + tb;
-                opop = pb;
-                tid = (uint32_t)(rel - rows[tb].newcum[pb] + 1);
-#pragma unroll
-                for (int p = 0; p < P; p++) x[p] = (p == pb) ? rows[tb].newfreq[p] : 0.0f;
-                gf_mutid id;
-                id.origin_generation = ogen;
-                id.origin_population = opop;
-                id.origin_threadID = (int32_t)tid;
-                id.reserved = 0;
-                ids[slot] = id;
-            }
-        }
-        u32x4 rc[P];
-        bool keep = false;
-        const auto *crows = as_const(rows);
-        for (int t = 0; t < S.K; t++) {
-            const auto &R = crows[t];
-            const int g = S.g0 + 1 + t;
-            if (valid && born < t) {
-                const bool refresh = ((g & 3) == 0) || (t == born + 1);
-                generation_update<P>(x, R, g, refresh, rc, S.k0, S.k1, tid, ogen, opop, LinearSel(), sh_ncdf, sh_eneg);
-            }
-            keep = valid && born <= t && keep_rule<P>(x, R.present_mask, preserved);
-            const uint64_t b = __ballot(keep);
-            if (BATCH) {
-                wave_cnt += (uint64_t)__popcll(b);   /* wave-uniform: scalar registers */
-            } else {
-                if (lane == t) cnt += (uint64_t)__popcll(b);
-            }
-        }
-        if (masks) {
-            /* the compaction flags of the block's last generation (what a
-             * flag pass over the written state would compute) */
-            const uint64_t kb = __ballot(keep);
-            if (lane == 0) {
-                masks[(size_t)tile * (WG / 64) + w] = kb;
-                sh_wc[w] = (unsigned int)__popcll(kb);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                unsigned int c = 0;
-                for (int k = 0; k < WG / 64; k++) c += sh_wc[k];
-                tile_counts[tile] = c;
-            }
-        }
-        if (valid) {
-#pragma unroll
-            for (int p = 0; p < P; p++) freq[p * pitch + slot] = x[p];
-        }
-    }
-    if (BATCH) {
-        /* every segment counts into the same 64 counters (seg0.live): one
-         * atomic per wave at the end, spread over 64 addresses (a per-tile
-         * atomic on one address held every wave at the next barrier) */
-        if (lane == 0 && wave_cnt) atomicAdd(&seg0.live[blockIdx.x & 63], (unsigned long long)wave_cnt);
-    } else {
-        __shared__ unsigned long long sh[64];
-        if (threadIdx.x < 64) sh[threadIdx.x] = 0;
-        __syncthreads();
-        if (cnt) atomicAdd(&sh[lane], (unsigned long long)cnt);
-        __syncthreads();
-        const SegLaunch<P> &S = seg0;
-        if ((int)threadIdx.x < S.K && sh[threadIdx.x])
-            atomicAdd(&S.live[(size_t)(S.t0 + threadIdx.x) * 64 + (blockIdx.x & 63)], sh[threadIdx.x]);
-    }
-}
+ * which measured 5 % faster than 7 waves with no spills (and 6 % faster
+ * than 6 waves without spills for three populations). */
 
 /* ---------------- fused advance + tile-local compaction ----------------
  * One launch advances every slot by K generations and writes the slots to

@@ -261,50 +261,6 @@ static __global__ __launch_bounds__(1024) void k_scan_tiles(const unsigned int *
     if (threadIdx.x == T - 1) seg_total[s] = (int)sh[T - 1];
 }
 
-/* scatter pass: write survivors in order into the new arrays; preserved
- * survivors get threadID = -|threadID| (reference scatter_arrays,
- * go_fish_impl.cu:147-189).  Batched: every segment's flagged survivors go
- * to the segment's new base.  A workgroup takes SCATTER_TILES consecutive
- * tiles: one binary search for the first tile's segment, then a linear
- * step to each next tile's (tiles are ordered by segment). */
-constexpr int SCATTER_TILES = 4;
-
-template <int P>
-__global__ __launch_bounds__(WG) void k_scatter(const float *__restrict__ freq, int64_t pitch,
-                                                const gf_mutid *__restrict__ ids,
-                                                float *__restrict__ nfreq, int64_t npitch,
-                                                gf_mutid *__restrict__ nids,
-                                                const int64_t *__restrict__ seg_base,
-                                                const int64_t *__restrict__ seg_nbase,
-                                                const int *__restrict__ seg_preserve,
-                                                const int *__restrict__ tile_first, int nseg,
-                                                const unsigned long long *__restrict__ masks,
-                                                const unsigned int *__restrict__ tile_off, int num_tiles) {
-    const int t0 = (int)blockIdx.x * SCATTER_TILES;
-    const int w = threadIdx.x >> 6;
-    int s = seg_of_tile(tile_first, nseg, t0);
-    for (int q = 0; q < SCATTER_TILES; q++) {
-        const int tile = t0 + q;
-        if (tile >= num_tiles) break;
-        while (s + 1 < nseg && tile_first[s + 1] <= tile) s++;   /* the last segment starting at or before the tile */
-        const uint64_t b = masks[(size_t)tile * (WG / 64) + w];
-        if (!((b >> (threadIdx.x & 63)) & 1ull)) continue;
-        const int li = (tile - tile_first[s]) * WG + (int)threadIdx.x;
-        unsigned int pos = tile_off[tile];
-        for (int k = 0; k < w; k++) pos += (unsigned int)__popcll(masks[(size_t)tile * (WG / 64) + k]);
-        pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-        const int64_t src = seg_base[s] + li;
-        const int64_t dst = seg_nbase[s] + pos;
-#pragma unroll
-        for (int p = 0; p < P; p++) nfreq[p * npitch + dst] = freq[p * pitch + src];
-        gf_mutid id = ids[src];
-        if (seg_preserve[s]) {
-            int z = id.origin_threadID;
-            id.origin_threadID = -(z < 0 ? -z : z);
-        }
-        nids[dst] = id;
-    }
-}
 
 /* ---------------- mutation-selection-equilibrium initialisation ---------------- */
 
