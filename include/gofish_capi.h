@@ -294,6 +294,12 @@ int32_t gf_sg_train(gf_sg_model *model, const float *x, const float *y, const fl
 /* ensemble mean SFS [n][99], mean log-total [n], disagreement [n] */
 int32_t gf_sg_predict(gf_sg_model *model, const float *x, int32_t n, float *sfs, float *log_total,
                       float *disagreement);
+/* gf_sg_predict with x, sfs, log_total and disagreement in device memory
+ * of the current device (log_total and disagreement may be NULL); returns
+ * when the outputs are written.  The caller's writes to x must be complete
+ * (the model's stream does not wait for other streams). */
+int32_t gf_sg_predict_device(gf_sg_model *model, const float *x, int32_t n, float *sfs, float *log_total,
+                             float *disagreement);
 /* raw logits of every member, n a multiple of 64: [ensemble][n][128] */
 int32_t gf_sg_forward(gf_sg_model *model, const float *x, int32_t n, float *logits);
 

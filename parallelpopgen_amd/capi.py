@@ -108,6 +108,7 @@ SIGNATURES = {
     "gf_sg_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                 C.c_float, C.c_float, C.c_void_p, C.c_void_p]),
     "gf_sg_predict": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gf_sg_predict_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gf_sg_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "gf_device_math": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.c_int32]),
@@ -155,6 +156,71 @@ def check(rc):
     if rc != GF_OK:
         raise GoFishError(rc, load().gf_last_error().decode())
     return rc
+
+
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime the library is linked against (opened by its soname
+    after the library, so it is the instance the library uses), with the
+    few calls DeviceArray needs."""
+    global _hip
+    if _hip is None:
+        load()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        for f in (h.hipMalloc, h.hipFree, h.hipMemcpy):
+            f.restype = C.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """An array in device memory of the current device, allocated through
+    the library's own HIP runtime, for the entry points that take device
+    pointers (tests and benchmarks).  `ptr` is the device address."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        rc = hip_runtime().hipMalloc(C.byref(p), max(self.nbytes, 1))
+        if rc != 0:
+            raise RuntimeError("hipMalloc of %d bytes failed (%d)" % (self.nbytes, rc))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            rc = hip_runtime().hipMemcpy(d.ptr, a.ctypes.data, d.nbytes, 1)   # host to device
+            if rc != 0:
+                raise RuntimeError("hipMemcpy to the device failed (%d)" % rc)
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            rc = hip_runtime().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2)   # device to host
+            if rc != 0:
+                raise RuntimeError("hipMemcpy from the device failed (%d)" % rc)
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            hip_runtime().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def _ptr(a, ctype):

@@ -58,11 +58,13 @@ struct Layout {
  * sqrt(2/pi).  With sigma(v) = 1 / (1 + e^-v), 0.5 (1 + tanh(u)) =
  * sigma(2u), so gelu(x) = x sigma(2u), evaluated with the hardware exp2 and
  * reciprocal (about 1e-6 relative; the activations are then rounded to
- * bfloat16).  e^-2u overflows to inf for u << 0, where sigma -> 0. */
+ * bfloat16).  e^-2u = 2^(x (k1 + k2 x^2)) with k1 = -2 c log2(e) and k2 =
+ * 0.044715 k1: five operations besides exp2 and the reciprocal.  e^-2u
+ * overflows to inf for u << 0, where sigma -> 0. */
 __device__ __forceinline__ float sigma2u(float x) {
-    const float c = 0.7978845608028654f;
-    const float u = c * (x + 0.044715f * x * x * x);
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+    const float k1 = -2.302208198f, k2 = -0.1029432396f;
+    const float t = x * __builtin_fmaf(k2, x * x, k1);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
 
 __device__ __forceinline__ float gelu(float x) { return x * sigma2u(x); }
@@ -157,9 +159,11 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
  * (the forward pass and prediction at batches of thousands of rows).  A
  * 128 x 128 tile per workgroup of 4 waves, each wave a 64 x 64 quadrant of
  * 2 x 2 blocks of v_mfma_f32_32x32x16_bf16; K in stages of 64 through two
- * LDS buffers: the next stage's global loads are issued before the current
- * stage's MFMAs and stored to the other buffer after them, one barrier per
- * stage.  Each output element accumulates its k = 0, 16, 32, ... products
+ * LDS buffers, fed from two register sets: stage s + 2's global loads are
+ * issued before stage s's MFMAs, and stage s + 1 (loaded during stage
+ * s - 1) is stored to the other buffer after them, one barrier per stage.
+ * The register sets and accumulators take 231 VGPRs, held to two waves per
+ * SIMD.  Each output element accumulates its k = 0, 16, 32, ... products
  * in the same order as k_gemm, so both kernels give the same bits.
  *
  * Tiles are dealt XCD-aware: the hardware hands workgroup b to XCD b % 8,
@@ -173,7 +177,7 @@ constexpr int GR = GK / 8;     /* threads per tile row in the global -> LDS copy
 constexpr int GRS = 256 / GR;  /* rows one pass of the workgroup copies */
 constexpr int GP = GT / GRS;   /* passes per operand and stage */
 
-__global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs g, int MT, int NT, int T) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const GemmArgs g, int MT, int NT, int T) {
     /* operand stages; after the K loop the same LDS stages the output tile */
     __shared__ __attribute__((aligned(16))) bf16 sm[2][2][GT][GLDS];
     auto &As = sm[0];
@@ -193,59 +197,88 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs g, int MT, int N
     const bf16 *pa = A + (int64_t)(m0 + lr) * g.lda + lk;
     const bf16 *pb = B + (int64_t)(n0 + lr) * g.ldb + lk;
     const int64_t da = GRS * g.lda, db = GRS * g.ldb;
-    bf16x8 ra[GP], rb[GP];
+    /* stage s + 1's operands wait in registers while stage s is in LDS:
+     * two register sets, loads issued two stages ahead */
+    bf16x8 ra[2][GP], rb[2][GP];
+    const int ns = g.K / GK;
+    auto load = [&](bf16x8 (&xa)[GP], bf16x8 (&xb)[GP], int k1) {
 #pragma unroll
-    for (int i = 0; i < GP; i++) {
-        ra[i] = *(const bf16x8 *)(pa + i * da);
-        rb[i] = *(const bf16x8 *)(pb + i * db);
-    }
+        for (int i = 0; i < GP; i++) {
+            xa[i] = *(const bf16x8 *)(pa + i * da + k1);
+            xb[i] = *(const bf16x8 *)(pb + i * db + k1);
+        }
+    };
+    auto store = [&](const bf16x8 (&xa)[GP], const bf16x8 (&xb)[GP], int buf) {
 #pragma unroll
-    for (int i = 0; i < GP; i++) {
-        *(bf16x8 *)&As[0][lr + GRS * i][lk] = ra[i];
-        *(bf16x8 *)&Bs[0][lr + GRS * i][lk] = rb[i];
-    }
-    __syncthreads();
+        for (int i = 0; i < GP; i++) {
+            *(bf16x8 *)&As[buf][lr + GRS * i][lk] = xa[i];
+            *(bf16x8 *)&Bs[buf][lr + GRS * i][lk] = xb[i];
+        }
+    };
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x16{};
-    const int ns = g.K / GK;
     const int fr = lane & 31, fk = 8 * (lane >> 5);
-    for (int s = 0; s < ns; s++) {
-        const int cur = s & 1;
-        const bool more = s + 1 < ns;
-        if (more) {
-            const int k1 = (s + 1) * GK;
-#pragma unroll
-            for (int i = 0; i < GP; i++) {
-                ra[i] = *(const bf16x8 *)(pa + i * da + k1);
-                rb[i] = *(const bf16x8 *)(pb + i * db + k1);
-            }
-        }
+    auto compute = [&](int buf) {
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
             bf16x8 fa[2], fb[2];
 #pragma unroll
-            for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8 *)&As[cur][wm * 64 + i * 32 + fr][kk + fk];
+            for (int i = 0; i < 2; i++) fa[i] = *(const bf16x8 *)&As[buf][wm * 64 + i * 32 + fr][kk + fk];
 #pragma unroll
-            for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8 *)&Bs[cur][wn * 64 + j * 32 + fr][kk + fk];
+            for (int j = 0; j < 2; j++) fb[j] = *(const bf16x8 *)&Bs[buf][wn * 64 + j * 32 + fr][kk + fk];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
                 for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
-        if (more) {
-            /* nobody reads buffer cur ^ 1 in this stage: it was last read
-             * in stage s - 1, before the barrier that ended it */
-#pragma unroll
-            for (int i = 0; i < GP; i++) {
-                *(bf16x8 *)&As[cur ^ 1][lr + GRS * i][lk] = ra[i];
-                *(bf16x8 *)&Bs[cur ^ 1][lr + GRS * i][lk] = rb[i];
-            }
-        }
+    };
+    /* every load of the steady-state loop is unconditional, so that the
+     * compiler's wait before a store counts only the older register set
+     * (a load under a branch makes it wait for both) */
+    load(ra[0], rb[0], 0);
+    load(ra[1], rb[1], ns > 1 ? GK : 0);   /* one stage: a harmless reload */
+    store(ra[0], rb[0], 0);
+    __syncthreads();
+    /* at the top of each pass: stage s in LDS buffer 0, stage s + 1 in
+     * flight into register set 1, set 0 free (its stage was stored in the
+     * previous pass).  A buffer is written only after the barrier that
+     * ended the last stage reading it. */
+    int s = 0;
+    for (; s + 3 < ns; s += 2) {
+        load(ra[0], rb[0], (s + 2) * GK);
+        __builtin_amdgcn_sched_barrier(0);   /* keep the loads ahead of the MFMAs */
+        compute(0);
+        store(ra[1], rb[1], 1);
+        __syncthreads();
+        load(ra[1], rb[1], (s + 3) * GK);
+        __builtin_amdgcn_sched_barrier(0);   /* keep the loads ahead of the MFMAs */
+        compute(1);
+        store(ra[0], rb[0], 0);
         __syncthreads();
     }
+    const int left = ns - s;   /* 1, 2 or 3 stages */
+    if (left == 3) {
+        load(ra[0], rb[0], (s + 2) * GK);
+        __builtin_amdgcn_sched_barrier(0);   /* keep the loads ahead of the MFMAs */
+        compute(0);
+        store(ra[1], rb[1], 1);
+        __syncthreads();
+        compute(1);
+        store(ra[0], rb[0], 0);
+        __syncthreads();
+        compute(0);
+    } else if (left == 2) {
+        compute(0);
+        store(ra[1], rb[1], 1);
+        __syncthreads();
+        compute(1);
+    } else {
+        compute(0);
+    }
+    __syncthreads();
     /* epilogue: accumulator r of lane l is C[8(r/4) + 4(l/32) + r%4][l%32]
      * of its block.  The tile goes through LDS (free after the loop's last
      * barrier) so that global stores are whole 16-byte row pieces: a lane's
@@ -411,12 +444,13 @@ __global__ void k_gather(const float *__restrict__ X, const float *__restrict__ 
     if (threadIdx.x == 0) t[b] = T[src];
 }
 
-/* inputs [B][6] fp32 -> [B][64] bf16, zero padded */
-__global__ void k_pack_inputs(const float *__restrict__ x, int B, int nin, bf16 *__restrict__ out) {
+/* inputs [nvalid][6] fp32 -> [B][64] bf16, zero padded (columns past nin,
+ * rows past nvalid) */
+__global__ void k_pack_inputs(const float *__restrict__ x, int B, int nvalid, int nin, bf16 *__restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)B * IN_PAD) return;
     const int b = (int)(i / IN_PAD), k = (int)(i % IN_PAD);
-    out[i] = (bf16)(k < nin ? x[(int64_t)b * nin + k] : 0.0f);
+    out[i] = (bf16)(k < nin && b < nvalid ? x[(int64_t)b * nin + k] : 0.0f);
 }
 
 /* ensemble statistics of the predictions: per-model softmax, mean SFS,
@@ -463,8 +497,8 @@ __global__ __launch_bounds__(256) void k_predict_stats(const float *__restrict__
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) worst = fmaxf(worst, __shfl_xor(worst, off));
     if (lane == 0) {
-        tot[b] = tsum / E;
-        dis[b] = worst;
+        if (tot) tot[b] = tsum / E;
+        if (dis) dis[b] = worst;
     }
 }
 
@@ -614,7 +648,7 @@ static int load_batch(gf_sg_model &S, const float *x, const float *y, const floa
     GF_HIP(hipMemcpyAsync(S.xin.p, x, (size_t)B * 6 * sizeof(float), hipMemcpyHostToDevice, S.st));
     if (y) GF_HIP(hipMemcpyAsync(S.y.p, y, (size_t)B * NBINS * sizeof(float), hipMemcpyHostToDevice, S.st));
     if (t) GF_HIP(hipMemcpyAsync(S.tot.p, t, (size_t)B * sizeof(float), hipMemcpyHostToDevice, S.st));
-    hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S.st, S.xin.p, B, 6, S.x.p);
+    hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S.st, S.xin.p, B, B, 6, S.x.p);
     GF_HIP(hipGetLastError());
     return GF_OK;
 }
@@ -742,7 +776,7 @@ int32_t gf_sg_train(gf_sg_model *S, const float *x, const float *y, const float 
                 GF_HIP(hipMemcpyAsync(S->y.p, dy.p + (size_t)b * B * NBINS, (size_t)B * NBINS * sizeof(float), hipMemcpyDeviceToDevice, S->st));
                 GF_HIP(hipMemcpyAsync(S->tot.p, dt.p + (size_t)b * B, (size_t)B * sizeof(float), hipMemcpyDeviceToDevice, S->st));
             }
-            hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S->st, S->xin.p, B, 6, S->x.p);
+            hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S->st, S->xin.p, B, B, 6, S->x.p);
             GF_TRY(train_step(*S, B, lr, w_tot, true));
         }
         GF_HIP(hipMemcpyAsync(lossv.data(), S->loss.p, S->E * sizeof(float), hipMemcpyDeviceToHost, S->st));
@@ -804,6 +838,40 @@ int32_t gf_sg_predict(gf_sg_model *S, const float *x, int32_t n, float *sfs, flo
         }
     }
     GF_HIP(hipStreamSynchronize(S->cst));
+    GF_HIP(hipStreamSynchronize(S->st));
+    return GF_OK;
+}
+
+/* gf_sg_predict on device memory: x, sfs, log_total and disagreement are
+ * device pointers on the model's device (the last two may be NULL); the
+ * batches run back to back on the model's stream, each writing its rows of
+ * the outputs, and the call returns when they are done. */
+int32_t gf_sg_predict_device(gf_sg_model *S, const float *x, int32_t n, float *sfs, float *tot, float *dis) {
+    if (!S || !x || n <= 0 || !sfs) { set_error("gf_sg_predict_device: invalid arguments"); return GF_ERR_ARG; }
+    int dev = -1;
+    GF_HIP(hipGetDevice(&dev));
+    for (const void *p : {(const void *)x, (const void *)sfs, (const void *)tot, (const void *)dis}) {
+        if (!p) continue;
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != dev) {
+            (void)hipGetLastError();
+            set_error("gf_sg_predict_device: x, sfs, log_total and disagreement must be memory of the current device");
+            return GF_ERR_ARG;
+        }
+    }
+    StreamScope scope(S->st);
+    const int B = std::min(65536, ((n + GT - 1) / GT) * GT);
+    GF_TRY(ensure_batch(*S, B));
+    for (int off = 0; off < n; off += B) {
+        const int cnt = std::min(B, n - off);
+        hipLaunchKernelGGL(k_pack_inputs, dim3((B * IN_PAD + 255) / 256), dim3(256), 0, S->st, x + (size_t)off * 6, B,
+                           cnt, 6, S->x.p);
+        GF_HIP(hipGetLastError());
+        GF_TRY(forward(*S, B, false));
+        hipLaunchKernelGGL(k_predict_stats, dim3((cnt + 3) / 4), dim3(256), 0, S->st, S->logits.p, (int64_t)B * OUT_PAD,
+                           S->E, cnt, sfs + (size_t)off * NBINS, tot ? tot + off : nullptr, dis ? dis + off : nullptr);
+        GF_HIP(hipGetLastError());
+    }
     GF_HIP(hipStreamSynchronize(S->st));
     return GF_OK;
 }

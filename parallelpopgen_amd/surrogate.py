@@ -113,6 +113,17 @@ class Network:
         capi.check(self.lib.gf_sg_predict(self.h, _p(x), n, _p(sfs), _p(tot), _p(dis)))
         return sfs, tot, dis
 
+    def predict_device(self, x, sfs, tot=None, dis=None):
+        """predict on device memory: x [n][6] and the outputs sfs [n][99],
+        tot [n], dis [n] are float32 capi.DeviceArray (tot and dis
+        optional)"""
+        n = x.shape[0]
+        for a, shape in ((x, (n, 6)), (sfs, (n, NBINS)), (tot, (n,)), (dis, (n,))):
+            if a is not None and (a.shape != shape or a.dtype != np.float32):
+                raise ValueError("predict_device: float32 device arrays of shapes [n][6], [n][99], [n]")
+        ptr = lambda a: C.c_void_p(a.ptr) if a is not None else None
+        capi.check(self.lib.gf_sg_predict_device(self.h, ptr(x), n, ptr(sfs), ptr(tot), ptr(dis)))
+
 
 # ---------------- parameter draws, features, simulation ----------------
 
@@ -277,6 +288,24 @@ def evaluate(sur, n, seed):
         simulator_sfs_per_s=n / t_sim,
         surrogate_with_fallback_sfs_per_s=n / t_sur,
     )
+
+
+def prediction_rate_device(net, n=1 << 20, repeats=3, seed=0):
+    """ensemble predictions per second for n inputs already in device
+    memory, outputs written to device memory"""
+    rng = np.random.default_rng(seed)
+    x = capi.DeviceArray.from_host(features(draw(n, rng)))
+    sfs, tot, dis = capi.DeviceArray((n, NBINS)), capi.DeviceArray(n), capi.DeviceArray(n)
+    net.predict_device(x, sfs, tot, dis)   # the batch buffers are allocated here
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        net.predict_device(x, sfs, tot, dis)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    for a in (x, sfs, tot, dis):
+        a.free()
+    return n / best
 
 
 def prediction_rate(net, n=1 << 20, repeats=3, seed=0):

@@ -152,6 +152,35 @@ def test_predict_pipelined_batches_match_one_batch():
     net.close()
 
 
+def test_predict_device_matches_host_predict():
+    """gf_sg_predict_device (inputs and outputs in device memory, batches
+    of 65,536 rows, the last ragged) gives the bits of gf_sg_predict; the
+    optional outputs may be left out; host pointers are refused."""
+    import ctypes as C
+    from parallelpopgen_amd.capi import DeviceArray
+    net = SG.Network(ensemble=4, seed=23)
+    rng = np.random.default_rng(4)
+    n = (1 << 17) + 100
+    x = rng.uniform(-1, 1, (n, 6)).astype(np.float32)
+    sfs, tot, dis = net.predict(x)
+    xd = DeviceArray.from_host(x)
+    sd = DeviceArray.from_host(np.full((n, SG.NBINS), -1.0, np.float32))
+    td, dd = DeviceArray.from_host(np.full(n, -1.0, np.float32)), DeviceArray.from_host(np.full(n, -1.0, np.float32))
+    net.predict_device(xd, sd, td, dd)
+    assert np.array_equal(sd.to_host(), sfs) and np.array_equal(td.to_host(), tot)
+    assert np.array_equal(dd.to_host(), dis)
+    x1 = DeviceArray.from_host(x[:1000])
+    s1 = DeviceArray((1000, SG.NBINS))
+    net.predict_device(x1, s1)
+    assert np.array_equal(s1.to_host(), sfs[:1000])
+    host_out = np.empty((10, SG.NBINS), np.float32)
+    rc = net.lib.gf_sg_predict_device(net.h, C.c_void_p(xd.ptr), 10, host_out.ctypes.data_as(C.c_void_p), None, None)
+    assert rc != 0 and "device" in net.lib.gf_last_error().decode()
+    for a in (xd, sd, td, dd, x1, s1):
+        a.free()
+    net.close()
+
+
 def test_invalid_arguments_are_rejected():
     net = SG.Network(ensemble=1, seed=1)
     x, y, t = inputs(100, 6)      # not a multiple of 64
