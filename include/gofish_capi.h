@@ -331,10 +331,15 @@ int32_t gf_device_math(int32_t op, const void *in0, const void *in1, const void 
 
 /* Frees the device memory the library keeps for reuse on `device` (-1:
  * the current device).  Released device buffers are kept in a per-device
- * cache of at most 16 GiB instead of being returned with hipFree, which
+ * cache of at most 32 GiB instead of being returned with hipFree, which
  * waits for the whole device (no counterpart in the reference, which
  * allocates and frees on every compaction, go_fish_impl.cuh:377-428). */
 int32_t gf_release_device_cache(int32_t device);
+/* The cache's state on `device` (-1: the current device): out[0] bytes
+ * and out[1] blocks kept for reuse, out[2] bytes held from the device (in
+ * use + kept), out[3] hipMalloc and out[4] hipFree calls so far (no
+ * counterpart in the reference). */
+int32_t gf_device_cache_stats(int32_t device, int64_t *out);
 
 /* ----- runtime ----- */
 const char *gf_last_error(void);

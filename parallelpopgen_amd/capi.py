@@ -78,6 +78,7 @@ SIGNATURES = {
     "gf_result_free": (None, [C.c_void_p]),
     "gf_check_sim": (C.c_int32, [C.POINTER(gf_sim_desc)]),
     "gf_release_device_cache": (C.c_int32, [C.c_int32]),
+    "gf_device_cache_stats": (C.c_int32, [C.c_int32, C.c_void_p]),
     "gf_site_frequency_spectrum_noalloc": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
                                                       C.c_void_p]),
     "gf_population_histogram": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_int32]),
@@ -150,6 +151,15 @@ def missing_symbols():
     """C ABI symbols declared in include/gofish_capi.h but absent from the library."""
     lib = load()
     return [n for n in SIGNATURES if getattr(lib, n, None) is None]
+
+
+def device_cache_stats(device=-1):
+    """the library's device-buffer cache: kept bytes and blocks, bytes held
+    from the device, hipMalloc and hipFree calls"""
+    out = np.zeros(5, np.int64)
+    check(load().gf_device_cache_stats(device, out.ctypes.data_as(C.c_void_p)))
+    return dict(kept_bytes=int(out[0]), kept_blocks=int(out[1]), held_bytes=int(out[2]),
+                mallocs=int(out[3]), frees=int(out[4]))
 
 
 def check(rc):

@@ -101,14 +101,18 @@ struct StreamScope {
  * completes (no device-wide wait); a block whose stream has ended is free
  * at once; a block of unknown stream waits for the device at release.  An
  * allocation takes the smallest cached block of its size class up to twice
- * its size, preferring blocks whose event has completed.  At most kKeep
- * bytes (32 GiB, a ninth of an MI355X's memory) are kept per device (the
- * oldest blocks are freed first); blocks above kMaxBlock are freed at once; a failed hipMalloc empties the
+ * its size, preferring blocks whose event has completed.  At most a third
+ * of the device's memory (96 GiB on an MI355X; at least kKeepMin) is kept
+ * per device, the oldest blocks freed first: a workload whose buffers
+ * vary from call to call across size classes (batched runs of 8,192
+ * simulations hold about 30 GiB each) keeps one block per class it uses
+ * instead of freeing and allocating multi-GiB blocks on every call.
+ * Blocks above kMaxBlock are freed at once; a failed hipMalloc empties the
  * device's cache and tries again; gf_release_device_cache empties it on
  * request.  The cache lives until the process ends. */
 class DevCache {
 public:
-    static constexpr size_t kKeep = (size_t)32 << 30;
+    static constexpr size_t kKeepMin = (size_t)32 << 30;
     static constexpr size_t kMaxBlock = (size_t)16 << 30;
 
     static DevCache &get() {
@@ -177,6 +181,12 @@ public:
             flush(dev);
             e = hipMalloc(p, block);
         }
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> lock(mu_);
+            Dev &D = dev_(dev);
+            D.mallocs++;
+            D.held += block;
+        }
         return e;
     }
 
@@ -197,8 +207,8 @@ public:
             (void)hipDeviceSynchronize();
         }
         if (block > kMaxBlock) {
-            if (ev) { (void)hipEventSynchronize(ev); recycle(ev); }
-            (void)hipFree(p);
+            std::vector<Block> drop{{p, block, ev}};
+            free_blocks(drop, dev);
         } else {
             std::vector<Block> drop;
             {
@@ -206,13 +216,13 @@ public:
                 Dev &D = dev_(dev);
                 D.blocks.push_back({p, block, ev});
                 D.kept += block;
-                while (D.kept > kKeep && !D.blocks.empty()) {
+                while (D.kept > D.keep && !D.blocks.empty()) {
                     drop.push_back(D.blocks.front());
                     D.kept -= D.blocks.front().bytes;
                     D.blocks.erase(D.blocks.begin());
                 }
             }
-            free_blocks(drop);
+            free_blocks(drop, dev);
         }
         if (cur != dev) (void)hipSetDevice(cur);
     }
@@ -226,19 +236,45 @@ public:
             drop.swap(D.blocks);
             D.kept = 0;
         }
-        free_blocks(drop);
+        free_blocks(drop, dev);
+    }
+
+    /* device `dev`: bytes and blocks kept for reuse, bytes held from the
+     * device (in use + kept), hipMalloc and hipFree calls so far */
+    void stats(int dev, int64_t out[5]) {
+        std::lock_guard<std::mutex> lock(mu_);
+        const Dev &D = dev_(dev);
+        out[0] = (int64_t)D.kept;
+        out[1] = (int64_t)D.blocks.size();
+        out[2] = (int64_t)D.held;
+        out[3] = D.mallocs;
+        out[4] = D.frees;
     }
 
 private:
     struct Block { void *p; size_t bytes; hipEvent_t ev; };
-    struct Dev { std::vector<Block> blocks; size_t kept = 0; };
+    struct Dev {
+        std::vector<Block> blocks;
+        size_t kept = 0, held = 0;
+        size_t keep = 0;   /* 0: not yet set */
+        int64_t mallocs = 0, frees = 0;
+    };
     std::mutex mu_;
     std::vector<Dev> devs_;
     std::vector<hipEvent_t> spare_;   /* completed events for reuse */
 
     Dev &dev_(int d) {
         if ((size_t)d >= devs_.size()) devs_.resize((size_t)d + 1);
-        return devs_[(size_t)d];
+        Dev &D = devs_[(size_t)d];
+        if (!D.keep) {
+            size_t total = 0;
+            if (hipDeviceTotalMem(&total, d) != hipSuccess) {
+                (void)hipGetLastError();
+                total = 0;
+            }
+            D.keep = std::max(kKeepMin, total / 3);
+        }
+        return D;
     }
 
     hipEvent_t event() {
@@ -263,10 +299,14 @@ private:
         spare_.push_back(e);
     }
 
-    void free_blocks(std::vector<Block> &drop) {
+    void free_blocks(std::vector<Block> &drop, int dev) {
         for (const Block &b : drop) {
             if (b.ev) { (void)hipEventSynchronize(b.ev); recycle(b.ev); }
             (void)hipFree(b.p);
+            std::lock_guard<std::mutex> lock(mu_);
+            Dev &D = dev_(dev);
+            D.frees++;
+            D.held -= b.bytes;
         }
         drop.clear();
     }

@@ -796,6 +796,18 @@ int32_t gf_release_device_cache(int32_t device) {
     return GF_OK;
 }
 
+int32_t gf_device_cache_stats(int32_t device, int64_t *out) {
+    if (!out) { gf::set_error("gf_device_cache_stats: null output"); return GF_ERR_ARG; }
+    int cur = 0;
+    GF_HIP(hipGetDevice(&cur));
+    GF_TRY(gf::select_device(device));
+    int dev = 0;
+    GF_HIP(hipGetDevice(&dev));
+    gf::DevCache::get().stats(dev, out);
+    if (dev != cur) GF_HIP(hipSetDevice(cur));
+    return GF_OK;
+}
+
 int32_t gf_device_count(int32_t *out) {
     int n = 0;
     GF_HIP(hipGetDeviceCount(&n));

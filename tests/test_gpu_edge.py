@@ -123,3 +123,23 @@ def test_release_device_cache(gpu, oracle):
     assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
     assert lib.gf_release_device_cache(-1) == 0
     assert_same(gpu.run(d), a, "after releasing the cache")
+
+
+def test_device_cache_statistics(gpu):
+    """gf_device_cache_stats: a run's released buffers stay kept (bytes held
+    from the device >= bytes kept), a second run of the same shape takes
+    them again without a new hipMalloc, and a release returns them all
+    (kept 0, one hipFree per kept block)."""
+    d = S.plain_small(G=60)
+    assert gpu.load().gf_release_device_cache(-1) == 0
+    gpu.run(d)
+    a = gpu.device_cache_stats()
+    assert a["kept_blocks"] > 0 and a["held_bytes"] >= a["kept_bytes"] > 0
+    gpu.run(d)
+    b = gpu.device_cache_stats()
+    assert b["mallocs"] == a["mallocs"] and b["frees"] == a["frees"]
+    assert gpu.load().gf_release_device_cache(-1) == 0
+    c = gpu.device_cache_stats()
+    assert c["kept_bytes"] == 0 and c["kept_blocks"] == 0
+    assert c["frees"] == b["frees"] + b["kept_blocks"]
+    assert c["held_bytes"] == b["held_bytes"] - b["kept_bytes"]
