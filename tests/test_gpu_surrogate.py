@@ -181,6 +181,31 @@ def test_predict_device_matches_host_predict():
     net.close()
 
 
+@pytest.mark.parametrize("n", [1, 127, 129, 4095, 4097])
+def test_predict_device_ragged_counts(n):
+    """Counts below, at and just past the 128-row tile and the 4,096-row
+    switch to the large-tile kernel: device and host prediction agree bit
+    for bit and the rows past n are not written."""
+    from parallelpopgen_amd.capi import DeviceArray
+    net = SG.Network(ensemble=2, seed=24)
+    x = np.random.default_rng(n).uniform(-1, 1, (n, 6)).astype(np.float32)
+    sfs, tot, dis = net.predict(x)
+    xd = DeviceArray.from_host(x)
+    guard = np.full((n + 64, SG.NBINS), 7.0, np.float32)
+    sd = DeviceArray.from_host(guard)
+    td = DeviceArray.from_host(np.full(n, -1.0, np.float32))
+    dd = DeviceArray.from_host(np.full(n, -1.0, np.float32))
+    rc = net.lib.gf_sg_predict_device(net.h, SG.C.c_void_p(xd.ptr), n, SG.C.c_void_p(sd.ptr),
+                                      SG.C.c_void_p(td.ptr), SG.C.c_void_p(dd.ptr))
+    assert rc == 0
+    out = sd.to_host()
+    assert np.array_equal(out[:n], sfs) and np.all(out[n:] == 7.0)
+    assert np.array_equal(td.to_host(), tot) and np.array_equal(dd.to_host(), dis)
+    for a in (xd, sd, td, dd):
+        a.free()
+    net.close()
+
+
 def test_invalid_arguments_are_rejected():
     net = SG.Network(ensemble=1, seed=1)
     x, y, t = inputs(100, 6)      # not a multiple of 64
