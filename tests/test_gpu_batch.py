@@ -151,3 +151,38 @@ def test_batch_growth_relayouts(gpu, scale):
         assert b["num_mutations"][k] == last["num_mutations"], k
         assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), k
         assert np.array_equal(ids, r["ids"]), k
+
+
+@pytest.mark.parametrize("interval", [8, 35])
+def test_random_batch_bit_identical_to_plain(gpu, interval):
+    """A seeded random batch of 96 simulations over a wider range than the
+    sweep: sizes 40 to 5,000 (below 100 the drift is the exact binomial
+    inversion), instantaneous size changes, F from 0 to 1 (haploid F = 1
+    included), h from 0 to 1, gamma from -20 to 10, 0 to 300 generations;
+    every simulation's final state and IDs equal gf_run's bit for bit."""
+    rng = np.random.default_rng(4242 + interval)
+    n = 96
+    sims = np.zeros(n, BATCH_SIM_DTYPE)
+    N = np.exp(rng.uniform(np.log(40), np.log(5000), n)).astype(np.int32)
+    F = rng.choice([0.0, 1.0, 0.5], n).astype(np.float32)
+    F[::3] = rng.uniform(0, 1, len(F[::3])).astype(np.float32)
+    sims["N"] = N
+    sims["N1"] = np.where(rng.uniform(size=n) < 0.3, np.maximum(40, N * np.exp(rng.uniform(-1, 2, n))), N).astype(np.int32)
+    sims["F"] = F
+    sims["h"] = rng.choice([0.0, 0.5, 1.0, 0.25], n)
+    sims["mu"] = np.float32(1e-8)
+    gamma = rng.uniform(-20, 10, n)
+    sims["s"] = [S.selection_from_gamma(g, Ni, Fi) for g, Ni, Fi in zip(gamma, N, F)]
+    sims["s"][::4] = 0.0
+    sims["num_sites"] = (rng.uniform(20, 200, n) / (2e-8 * np.maximum(N, sims["N1"]))).astype(np.float32)
+    sims["seed1"] = rng.integers(1, 2 ** 31 - 1, n)
+    sims["seed2"] = rng.integers(1, 2 ** 31 - 1, n)
+    sims["num_generations"] = rng.integers(0, 300, n)
+    b = gpu.batch_run(sims, compact_interval=interval, sfs_n=30, keep_states=True)
+    for k, sim in enumerate(sims):
+        r = gpu.run(S.sweep_plain_desc(sim, interval))
+        last = r["samples"][-1]
+        f, ids = b["states"][k]
+        assert b["num_mutations"][k] == last["num_mutations"], k
+        assert np.array_equal(f.view(np.uint32), last["freq"][0].view(np.uint32)), k
+        assert np.array_equal(ids, r["ids"]), k
