@@ -84,7 +84,7 @@ def parse():
     ap.add_argument("--workload", choices=["plain_large", "plain_small", "three_population", "batched", "surrogate", "speed"],
                     default="plain_large")
     ap.add_argument("--sims", type=int, default=4096, help="batched: simulations in the sweep")
-    ap.add_argument("--plain-sample", type=int, default=32, help="batched: simulations timed back to back")
+    ap.add_argument("--plain-sample", type=int, default=256, help="batched: simulations timed back to back")
     ap.add_argument("--train-draws", type=int, default=1000000, help="surrogate: training draws")
     ap.add_argument("--test-draws", type=int, default=10000, help="surrogate: held-out draws")
     ap.add_argument("--plain-only", action="store_true",
